@@ -1,0 +1,221 @@
+/*
+ * stablebear_amd: MI355X-native (gfx950) pairwise PCF integration.
+ *
+ * C-ABI drop-in for the reference's GPU plug-in point. Plain C types only; no
+ * C++ types or exceptions cross this boundary. Every function returns an int
+ * status (SBAMD_OK on success); on failure sbamd_last_error() gives a
+ * thread-local message.
+ *
+ * Data model. A PCF set is CSR: offsets[n + 1] (int64, host), and the
+ * breakpoint times t[offsets[n]] and values v[offsets[n]] in one dtype
+ * (SBAMD_F32 for the reference's pcf32, SBAMD_F64 for pcf64). PCF i takes the
+ * value v[k] on [t[k], t[k+1]) and v[last] on [t[last], inf). Validation
+ * follows the reference's Pcf constructor (src/python/pypcf_support.hpp:36-89):
+ * times non-decreasing, first time exactly 0; an empty PCF is the zero PCF
+ * [(0, 0)]. Times must also be finite.
+ *
+ * Output layouts (written by the device directly, 64-bit indices):
+ *   SBAMD_OUT_DISTMAT  D[i,j], i != j, at max(i,j)*(max(i,j)-1)/2 + min(i,j)
+ *                      (include/sbear/distance_matrix.hpp:120-146)
+ *   SBAMD_OUT_SYMMAT   K[i,j] at max(i,j)*(max(i,j)+1)/2 + min(i,j)
+ *                      (include/sbear/symmetric_matrix.hpp:80-103)
+ *   SBAMD_OUT_DENSE    X[i,j] at i * n_cols + j (py_distance.cpp:156-160)
+ */
+#ifndef STABLEBEAR_AMD_H
+#define STABLEBEAR_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define SBAMD_OK 0
+#define SBAMD_ERR_INVALID 1   /* bad argument or input (maps to ValueError) */
+#define SBAMD_ERR_RUNTIME 2   /* HIP / runtime failure (maps to RuntimeError) */
+#define SBAMD_ERR_NODEVICE 3  /* no HIP device */
+#define SBAMD_ERR_CANCELLED 4 /* job was cancelled */
+#define SBAMD_TIMEOUT 5       /* sbamd_job_wait: not finished yet */
+#define SBAMD_ERR_NOMEM 6     /* device or host allocation failed */
+
+/* ---- enums ---- */
+#define SBAMD_F32 1
+#define SBAMD_F64 2
+
+#define SBAMD_OP_L1 0  /* |f - g|, identity finalize      (operations.cuh:19-31) */
+#define SBAMD_OP_LP 1  /* |f - g|^p, finalize x^(1/p)      (operations.cuh:33-51) */
+#define SBAMD_OP_DOT 2 /* f * g, identity finalize         (operations.cuh:53-65) */
+
+#define SBAMD_OUT_DISTMAT 0
+#define SBAMD_OUT_SYMMAT 1
+#define SBAMD_OUT_DENSE 2
+
+/* Arithmetic forms.
+ * FP64  : times and values in FP64, one FMA per segment in walk order; the
+ *         same operation sequence as the reference CPU integral.
+ * FP32C : breakpoint comparisons and segment lengths in the input time dtype,
+ *         values and segment products in FP32, Kahan-compensated FP32
+ *         accumulation per pair, final per-pair combination in FP64; pairs
+ *         whose walk spans more than the plan's fold threshold of segments
+ *         fold their partial sums into an FP64 accumulator at every phase. */
+#define SBAMD_PREC_FP64 0
+#define SBAMD_PREC_FP32C 1
+
+typedef struct sbamd_ctx sbamd_ctx;
+typedef struct sbamd_set sbamd_set;
+typedef struct sbamd_plan sbamd_plan;
+typedef struct sbamd_job sbamd_job;
+typedef struct sbamd_graph sbamd_graph;
+
+typedef struct
+{
+  int64_t n;               /* number of PCFs */
+  const int64_t* offsets;  /* n + 1 entries, host memory */
+  const void* t;           /* offsets[n] times, host memory */
+  const void* v;           /* offsets[n] values, host memory */
+  int32_t dtype;           /* SBAMD_F32 or SBAMD_F64 */
+} sbamd_pcf_set;
+
+/* Pair-tile shape of the lane-per-pair kernel: rows x cols pairs per
+ * workgroup (rows * cols <= 256), `cap` breakpoint records per PCF staged in
+ * LDS per phase. Zero fields mean "use the tuned or default value". */
+typedef struct
+{
+  int32_t rows;
+  int32_t cols;
+  int32_t cap;
+  int32_t reserved;
+} sbamd_tile_shape;
+
+typedef struct
+{
+  uint64_t pairs;          /* output entries computed */
+  uint64_t segments;       /* total union segments walked (exact, host count) */
+  uint64_t out_count;      /* output elements */
+  uint64_t tiles;          /* workgroup tiles */
+  uint64_t input_bytes;    /* device bytes of the staged record arrays */
+  int32_t rows, cols, cap; /* tile shape in use */
+  int32_t problems;        /* 1, or k for a batched plan */
+} sbamd_plan_stats;
+
+/* ---- status / device ---- */
+const char* sbamd_last_error(void);
+const char* sbamd_version(void);
+int sbamd_device_count(int* count);
+
+/* ---- context (one HIP device, one default stream) ----
+ * Replaces the per-device state of CudaBlockPipeline / PcfBlockOp
+ * (include/sbear/cuda/cuda_block_executor.cuh:108-341, pcf_block_op.cuh:32-123). */
+int sbamd_ctx_create(int device, sbamd_ctx** out);
+int sbamd_ctx_destroy(sbamd_ctx* ctx);
+int sbamd_ctx_synchronize(sbamd_ctx* ctx);
+
+/* ---- device-resident PCF sets ----
+ * Validates, drops zero-length segments, sorts PCFs by record count and
+ * uploads CSR record arrays laid out for `precision` (one H2D per set).
+ * Replaces init_pcf_data / OffsetDataManager / upload_subset
+ * (cuda_pcf_data_manager.cuh:13-23, offset_data_manager.hpp:42-68,
+ * cuda_offset_data_manager.cuh:24-44). */
+int sbamd_set_create(sbamd_ctx* ctx, const sbamd_pcf_set* set, int32_t precision, sbamd_set** out);
+int sbamd_set_destroy(sbamd_set* set);
+int sbamd_set_info(const sbamd_set* set, int64_t* n, int64_t* records, int64_t* points);
+/* Replace the values of an existing set from a device array laid out like the
+ * original v (offsets[n] entries, set dtype); times and structure unchanged. */
+int sbamd_set_update_values(sbamd_set* set, const void* v_device, void* stream);
+
+/* ---- plans: tile work list for one pairwise operation ----
+ * cols == NULL: pdist (DISTMAT) or l2_kernel (SYMMAT) over rows;
+ * cols != NULL: cdist (DENSE, n_rows x n_cols). Replaces CudaBlockScheduler +
+ * get_max_output_elements / get_min_block_side / get_grid_dims
+ * (cuda_block_scheduler.hpp:29-129, block_matrix_support.cuh:26-93). */
+int sbamd_plan_create(sbamd_ctx* ctx, const sbamd_set* rows, const sbamd_set* cols, int32_t op,
+                      double p, int32_t out_kind, const sbamd_tile_shape* shape, sbamd_plan** out);
+/* Batched pdist / l2_kernel over k independent sets in one launch; outputs
+ * are concatenated (problem b at sbamd_plan_out_offset(plan, b)). */
+int sbamd_plan_create_batched(sbamd_ctx* ctx, int32_t k, const sbamd_set* const* sets, int32_t op,
+                              double p, int32_t out_kind, const sbamd_tile_shape* shape,
+                              sbamd_plan** out);
+int sbamd_plan_destroy(sbamd_plan* plan);
+int sbamd_plan_info(const sbamd_plan* plan, sbamd_plan_stats* info);
+int sbamd_plan_out_offset(const sbamd_plan* plan, int32_t problem, uint64_t* offset);
+/* Pairs whose union walk spans at least this many segments fold FP32C
+ * partials into FP64 per phase (default 8192). */
+int sbamd_plan_set_fold_threshold(sbamd_plan* plan, int64_t segments);
+
+/* Enqueue the whole forward computation on `stream` (hipStream_t; NULL = the
+ * context stream). out_device: device buffer of plan out_count elements of
+ * out_dtype. row_sum_device (optional, DENSE only): n_rows FP64 sums of each
+ * output row (for row means). Asynchronous; capturable into a hipGraph.
+ * Replaces launch_pcf_block_integrate + the block pipeline's scatter
+ * (cuda_pcf_kernel.cuh:95-144, cuda_result_writer.hpp:46-108). */
+int sbamd_plan_execute(sbamd_plan* plan, void* out_device, int32_t out_dtype, void* row_sum_device,
+                       void* stream);
+
+/* Backward of L = sum_e grad_out[e] * out[e] over the plan's output entries
+ * (no reference counterpart). op must be SBAMD_OP_LP with p == 2 (fwd_out =
+ * the forward distances) or SBAMD_OP_DOT. grad_v / grad_t are device arrays
+ * indexed like the set's original points (FP32 or FP64 per grad_dtype),
+ * overwritten. Single-set plans only. */
+int sbamd_plan_backward(sbamd_plan* plan, const void* grad_out_device, const void* fwd_out_device,
+                        int32_t out_dtype, void* grad_v_device, void* grad_t_device,
+                        int32_t grad_dtype, void* stream);
+
+/* ---- hipGraph capture of a plan's forward launch ---- */
+int sbamd_graph_create(sbamd_plan* plan, void* out_device, int32_t out_dtype, sbamd_graph** out);
+int sbamd_graph_launch(sbamd_graph* graph, void* stream);
+int sbamd_graph_destroy(sbamd_graph* graph);
+
+/* ---- autotuner: times candidate tile shapes on this plan and keeps the best
+ * (also returned). Candidates are limited by the LDS budget. ---- */
+int sbamd_autotune(sbamd_plan* plan, void* out_device, int32_t out_dtype, int32_t reps,
+                   sbamd_tile_shape* best, double* best_ms);
+int sbamd_plan_set_shape(sbamd_plan* plan, const sbamd_tile_shape* shape);
+
+/* ---- asynchronous host-to-host jobs (the reference's StoppableTask) ----
+ * Runs set upload, planning, execution in chunks, and the download into
+ * out_host on a worker thread. progress counts pairs ("integral" unit).
+ * Replaces create_cuda_block_{integrate_l1,integrate_lp,integrate_l2_kernel,
+ * cdist_l1,cdist_lp}_task (cuda_matrix_integrate_api.hpp:19-113) and the
+ * StoppableTask wait / progress / cancel protocol (task.hpp:15-193). */
+int sbamd_pairwise_submit(sbamd_ctx* ctx, const sbamd_pcf_set* rows, const sbamd_pcf_set* cols,
+                          int32_t op, double p, int32_t out_kind, int32_t precision, void* out_host,
+                          int32_t out_dtype, sbamd_job** out);
+/* SBAMD_OK when finished, SBAMD_TIMEOUT if still running after timeout_ms
+ * (< 0 waits forever); the job's own error status once finished. */
+int sbamd_job_wait(sbamd_job* job, int64_t timeout_ms);
+int sbamd_job_progress(sbamd_job* job, uint64_t* done, uint64_t* total);
+int sbamd_job_cancel(sbamd_job* job);
+int sbamd_job_destroy(sbamd_job* job);
+
+/* ---- single-pair helpers (synchronous) ---- */
+/* lp_distance(f, g, p): lp_distance.hpp:10-21. f, g: one-PCF sets. */
+int sbamd_lp_distance(sbamd_ctx* ctx, const sbamd_pcf_set* f, const sbamd_pcf_set* g, double p,
+                      double* out);
+/* Rectangles of the walk over [a, b): iterate_rectangles.hpp:37-104. out
+ * holds 4 values (left, right, f_value, g_value) per rectangle in the set
+ * dtype; *count receives the total (only the first cap are written). */
+int sbamd_iterate_rectangles(sbamd_ctx* ctx, const sbamd_pcf_set* f, const sbamd_pcf_set* g,
+                             double a, double b, void* out, int64_t cap, int64_t* count);
+
+/* ---- host-only planning helpers (no device needed) ---- */
+/* Validate a host set like sbamd_set_create would; returns the status. */
+int sbamd_validate_set(const sbamd_pcf_set* set);
+/* Count union segments of the walk over all pairs of a pdist (cols == NULL,
+ * include_diagonal 0), l2_kernel (cols == NULL, include_diagonal 1) or cdist
+ * (cols != NULL), exactly, after zero-length segments are dropped. */
+int sbamd_count_segments(const sbamd_pcf_set* rows, const sbamd_pcf_set* cols,
+                         int32_t include_diagonal, uint64_t* segments);
+/* Tile list the planner would build for record counts m_rows (n_rows) and
+ * m_cols (n_cols, or NULL for the symmetric lower triangle): writes up to cap
+ * tiles as (row0, col0, nrows, ncols) int32 quadruples in sorted index space
+ * plus the sorted permutation of rows (perm_rows, n_rows entries). */
+int sbamd_plan_tiles_host(int64_t n_rows, const int64_t* m_rows, int64_t n_cols,
+                          const int64_t* m_cols, int32_t tile_rows, int32_t tile_cols,
+                          int32_t* tiles, int64_t cap, int64_t* n_tiles, int64_t* perm_rows);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,1253 @@
+// C-ABI implementation: contexts, device-resident sets, plans, asynchronous
+// jobs, graphs, autotuning and the single-pair helpers. No exception crosses
+// the boundary: every entry point converts failures into a status code and a
+// thread-local message.
+#include "sbamd_internal.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <type_traits>
+
+namespace sbamd
+{
+  [[noreturn]] void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+  void hip_check(hipError_t e, const char* what, const char* file, int line)
+  {
+    if (e == hipSuccess)
+      return;
+    std::string msg = std::string(file) + ":" + std::to_string(line) + ": " + what + ": " +
+                      hipGetErrorString(e);
+    fail(e == hipErrorOutOfMemory ? SBAMD_ERR_NOMEM : SBAMD_ERR_RUNTIME, msg);
+  }
+
+  DeviceBuffer::DeviceBuffer(size_t b) : bytes(b)
+  {
+    if (b > 0)
+      SBAMD_HIP(hipMalloc(&ptr, b));
+  }
+
+  DeviceBuffer& DeviceBuffer::operator=(DeviceBuffer&& o) noexcept
+  {
+    if (this != &o)
+    {
+      if (ptr)
+        (void)hipFree(ptr);
+      ptr = o.ptr;
+      bytes = o.bytes;
+      o.ptr = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+
+  DeviceBuffer::~DeviceBuffer()
+  {
+    if (ptr)
+      (void)hipFree(ptr);
+  }
+
+  namespace
+  {
+    thread_local std::string g_last_error;
+
+    template <typename F>
+    int guarded(F&& f)
+    {
+      try
+      {
+        f();
+        return SBAMD_OK;
+      }
+      catch (const Error& e)
+      {
+        g_last_error = e.msg;
+        return e.code;
+      }
+      catch (const std::bad_alloc&)
+      {
+        g_last_error = "host allocation failed";
+        return SBAMD_ERR_NOMEM;
+      }
+      catch (const std::exception& e)
+      {
+        g_last_error = e.what();
+        return SBAMD_ERR_RUNTIME;
+      }
+      catch (...)
+      {
+        g_last_error = "unknown error";
+        return SBAMD_ERR_RUNTIME;
+      }
+    }
+
+    void need(const void* p, const char* what)
+    {
+      if (!p)
+        fail(SBAMD_ERR_INVALID, std::string(what) + " is NULL");
+    }
+
+    void use_device(const Ctx* ctx) { SBAMD_HIP(hipSetDevice(ctx->device)); }
+
+    hipStream_t pick_stream(const Ctx* ctx, void* stream)
+    {
+      return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    }
+
+    template <typename T>
+    void upload(DeviceBuffer& buf, const std::vector<T>& v)
+    {
+      buf = DeviceBuffer(v.size() * sizeof(T));
+      if (!v.empty())
+        SBAMD_HIP(hipMemcpy(buf.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+
+    // Kernel op for (op, p): p == 2 takes the d*d / sqrt specialisation.
+    int kernel_op(int32_t op, double p)
+    {
+      if (op == SBAMD_OP_L1)
+        return OPK_L1;
+      if (op == SBAMD_OP_DOT)
+        return OPK_DOT;
+      if (op == SBAMD_OP_LP)
+        return p == 2.0 ? OPK_L2 : OPK_LP;
+      fail(SBAMD_ERR_INVALID, "unknown op");
+    }
+  }
+}
+
+using namespace sbamd;
+
+struct sbamd_ctx : Ctx
+{
+};
+struct sbamd_set : Set
+{
+};
+struct sbamd_plan : Plan
+{
+};
+
+namespace sbamd
+{
+  namespace
+  {
+    void check_shape(int R, int C, int cap)
+    {
+      if (R < 1 || C < 1 || R * C > 256 || cap < 2 || cap > 4096)
+        fail(SBAMD_ERR_INVALID, "tile shape out of range (rows*cols <= 256, 2 <= cap <= 4096)");
+      const int block = std::max(64, (R * C + 63) / 64 * 64);
+      if (R + C > block)
+        fail(SBAMD_ERR_INVALID, "tile shape needs rows + cols <= workgroup size");
+    }
+
+    void set_layout_for(int32_t precision, int32_t in_dtype, int32_t& tt_f64, int32_t& tv_f64)
+    {
+      if (precision == SBAMD_PREC_FP64)
+      {
+        tt_f64 = 1;
+        tv_f64 = 1;
+      }
+      else if (precision == SBAMD_PREC_FP32C)
+      {
+        tt_f64 = in_dtype == SBAMD_F64 ? 1 : 0;
+        tv_f64 = 0;
+      }
+      else
+        fail(SBAMD_ERR_INVALID, "unknown precision");
+    }
+
+    void make_set(Ctx* ctx, const sbamd_pcf_set& in, int32_t precision, Set& s)
+    {
+      PackedSet pk;
+      pack_set(in, pk);
+      s.ctx = ctx;
+      s.precision = precision;
+      s.in_dtype = in.dtype;
+      set_layout_for(precision, in.dtype, s.tt_f64, s.tv_f64);
+      s.n = pk.n;
+      s.records = pk.roff.empty() ? 0 : pk.roff.back();
+      s.points = pk.points;
+      s.h_roff = pk.roff;
+      s.h_perm = pk.perm;
+      s.h_src = pk.src;
+      use_device(ctx);
+      upload(s.roff, pk.roff);
+      upload(s.perm, pk.perm);
+      upload(s.src, pk.src);
+      if (s.tt_f64)
+        upload(s.tend, pk.tend64);
+      else
+      {
+        std::vector<float> tf(pk.tend64.begin(), pk.tend64.end());
+        upload(s.tend, tf);
+      }
+      if (s.tv_f64)
+        upload(s.val, pk.val64);
+      else
+      {
+        std::vector<float> vf(pk.val64.begin(), pk.val64.end());
+        upload(s.val, vf);
+      }
+    }
+
+    std::vector<int64_t> sorted_counts(const Set& s)
+    {
+      std::vector<int64_t> m(static_cast<size_t>(s.n));
+      for (int64_t q = 0; q < s.n; ++q)
+        m[q] = s.h_roff[q + 1] - s.h_roff[q];
+      return m;
+    }
+
+    uint64_t tri(uint64_t n) { return n * (n - 1) / 2; }
+
+    void finish_plan(Plan& pl)
+    {
+      use_device(pl.ctx);
+      upload(pl.probs, pl.h_probs);
+      upload(pl.tiles, pl.h_tiles);
+    }
+
+    FwdParams fwd_params(const Plan& pl, void* out, int32_t out_dtype, double* row_sum)
+    {
+      FwdParams P;
+      std::memset(&P, 0, sizeof(P));
+      const Set* rs = pl.sets[0];
+      const Set* cs = pl.cdist ? pl.sets[1] : pl.sets[0];
+      if (pl.batched)
+      {
+        P.r_off = pl.b_roff.as<int64_t>();
+        P.r_tend = pl.b_tend.ptr;
+        P.r_val = pl.b_val.ptr;
+        P.r_perm = pl.b_perm.as<int64_t>();
+        P.c_off = P.r_off;
+        P.c_tend = P.r_tend;
+        P.c_val = P.r_val;
+        P.c_perm = P.r_perm;
+      }
+      else
+      {
+        P.r_off = rs->roff.as<int64_t>();
+        P.r_tend = rs->tend.ptr;
+        P.r_val = rs->val.ptr;
+        P.r_perm = rs->perm.as<int64_t>();
+        P.c_off = cs->roff.as<int64_t>();
+        P.c_tend = cs->tend.ptr;
+        P.c_val = cs->val.ptr;
+        P.c_perm = cs->perm.as<int64_t>();
+      }
+      P.tiles = pl.tiles.as<Tile>();
+      P.probs = pl.probs.as<Problem>();
+      P.out = out;
+      P.row_sum = row_sum;
+      P.ntiles = static_cast<int64_t>(pl.h_tiles.size());
+      P.out_cols = pl.cdist ? pl.sets[1]->n : 0;
+      P.p = pl.p;
+      P.out_kind = pl.out_kind;
+      P.out_f64 = out_dtype == SBAMD_F64 ? 1 : 0;
+      P.R = pl.R;
+      P.C = pl.C;
+      P.cap = pl.cap;
+      P.stride = pl.R + pl.C;
+      P.symmetric = pl.cdist ? 0 : 1;
+      P.include_diag = pl.out_kind == SBAMD_OUT_SYMMAT ? 1 : 0;
+      return P;
+    }
+
+    void execute_range(const Plan& pl, void* out, int32_t out_dtype, double* row_sum, int64_t t0,
+                       int64_t t1, hipStream_t stream)
+    {
+      if (fwd_lds_bytes(pl.tt_f64, pl.tv_f64, pl.R, pl.C, pl.cap) > pl.ctx->lds_per_block)
+        fail(SBAMD_ERR_INVALID, "tile shape needs more LDS than a workgroup may use");
+      FwdParams P = fwd_params(pl, out, out_dtype, row_sum);
+      P.tiles = pl.tiles.as<Tile>() + t0;
+      P.ntiles = t1 - t0;
+      launch_forward(P, pl.tt_f64, pl.tv_f64, kernel_op(pl.op, pl.p),
+                     pl.precision == SBAMD_PREC_FP32C ? 1 : 0, stream);
+    }
+
+    void check_out_dtype(int32_t d)
+    {
+      if (d != SBAMD_F32 && d != SBAMD_F64)
+        fail(SBAMD_ERR_INVALID, "output dtype must be SBAMD_F32 or SBAMD_F64");
+    }
+
+    uint64_t tile_pairs(const Tile& t, bool include_diag)
+    {
+      if (t.flags & TILE_DIAG)
+      {
+        const uint64_t n = static_cast<uint64_t>(t.nr);
+        return include_diag ? n * (n + 1) / 2 : n * (n - 1) / 2;
+      }
+      return static_cast<uint64_t>(t.nr) * static_cast<uint64_t>(t.nc);
+    }
+  }
+
+  void Plan::rebuild_tiles()
+  {
+    h_tiles.clear();
+    const bool diag = out_kind == SBAMD_OUT_SYMMAT;
+    if (cdist)
+    {
+      auto mr = sorted_counts(*sets[0]);
+      auto mc = sorted_counts(*sets[1]);
+      build_tiles(mr, &mc, R, C, false, 0, fold_threshold, h_tiles);
+    }
+    else
+    {
+      for (size_t b = 0; b < sets.size(); ++b)
+      {
+        auto m = sorted_counts(*sets[b]);
+        build_tiles(m, nullptr, R, C, diag, static_cast<uint16_t>(b), fold_threshold, h_tiles);
+      }
+    }
+    if (batched)
+    {
+      // Longest tiles first across all problems (LPT order for the launch).
+      std::vector<std::pair<int64_t, size_t>> cost(h_tiles.size());
+      for (size_t k = 0; k < h_tiles.size(); ++k)
+      {
+        const Tile& t = h_tiles[k];
+        const Set& s = *sets[t.prob];
+        int64_t mr = 0, mc = 0;
+        for (int q = 0; q < t.nr; ++q)
+          mr = std::max(mr, s.h_roff[t.r0 + q + 1] - s.h_roff[t.r0 + q]);
+        for (int q = 0; q < t.nc; ++q)
+          mc = std::max(mc, s.h_roff[t.c0 + q + 1] - s.h_roff[t.c0 + q]);
+        cost[k] = {-(mr + mc), k};
+      }
+      std::stable_sort(cost.begin(), cost.end());
+      std::vector<Tile> sorted(h_tiles.size());
+      for (size_t k = 0; k < cost.size(); ++k)
+        sorted[k] = h_tiles[cost[k].second];
+      h_tiles.swap(sorted);
+    }
+    finish_plan(*this);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Jobs
+// ---------------------------------------------------------------------------
+struct sbamd_job
+{
+  std::thread worker;
+  std::mutex mu;
+  std::condition_variable cv;
+  bool done = false;
+  int status = SBAMD_OK;
+  std::string error;
+  std::atomic<uint64_t> progress{0};
+  std::atomic<uint64_t> total{0};
+  std::atomic<bool> cancel{false};
+};
+
+struct sbamd_graph
+{
+  Ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C"
+{
+  const char* sbamd_last_error(void) { return g_last_error.c_str(); }
+
+  const char* sbamd_version(void) { return "stablebear_amd 0.1.0 (gfx950)"; }
+
+  int sbamd_device_count(int* count)
+  {
+    return guarded([&] {
+      need(count, "count");
+      int n = 0;
+      hipError_t e = hipGetDeviceCount(&n);
+      if (e != hipSuccess)
+        n = 0;
+      *count = n;
+    });
+  }
+
+  int sbamd_ctx_create(int device, sbamd_ctx** out)
+  {
+    return guarded([&] {
+      need(out, "out");
+      *out = nullptr;
+      int n = 0;
+      if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        fail(SBAMD_ERR_NODEVICE, "no HIP device is available");
+      if (device < 0 || device >= n)
+        fail(SBAMD_ERR_INVALID, "device ordinal out of range");
+      auto ctx = std::make_unique<sbamd_ctx>();
+      ctx->device = device;
+      SBAMD_HIP(hipSetDevice(device));
+      hipDeviceProp_t prop;
+      SBAMD_HIP(hipGetDeviceProperties(&prop, device));
+      ctx->cu_count = prop.multiProcessorCount;
+      ctx->lds_per_block = prop.sharedMemPerBlock;
+      SBAMD_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+      *out = ctx.release();
+    });
+  }
+
+  int sbamd_ctx_destroy(sbamd_ctx* ctx)
+  {
+    return guarded([&] {
+      if (!ctx)
+        return;
+      use_device(ctx);
+      if (ctx->stream)
+      {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+      }
+      delete ctx;
+    });
+  }
+
+  int sbamd_ctx_synchronize(sbamd_ctx* ctx)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      use_device(ctx);
+      SBAMD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+  }
+
+  int sbamd_validate_set(const sbamd_pcf_set* set)
+  {
+    return guarded([&] {
+      need(set, "set");
+      validate_set(*set);
+    });
+  }
+
+  int sbamd_set_create(sbamd_ctx* ctx, const sbamd_pcf_set* set, int32_t precision, sbamd_set** out)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      need(set, "set");
+      need(out, "out");
+      *out = nullptr;
+      auto s = std::make_unique<sbamd_set>();
+      make_set(ctx, *set, precision, *s);
+      *out = s.release();
+    });
+  }
+
+  int sbamd_set_destroy(sbamd_set* set)
+  {
+    return guarded([&] {
+      if (!set)
+        return;
+      use_device(set->ctx);
+      delete set;
+    });
+  }
+
+  int sbamd_set_info(const sbamd_set* set, int64_t* n, int64_t* records, int64_t* points)
+  {
+    return guarded([&] {
+      need(set, "set");
+      if (n)
+        *n = set->n;
+      if (records)
+        *records = set->records;
+      if (points)
+        *points = set->points;
+    });
+  }
+
+  int sbamd_set_update_values(sbamd_set* set, const void* v_device, void* stream)
+  {
+    return guarded([&] {
+      need(set, "set");
+      need(v_device, "v_device");
+      use_device(set->ctx);
+      if (set->records > 0 && set->points == 0)
+        return; // only zero PCFs: nothing to refresh
+      for (int64_t r : set->h_src)
+        if (r < 0)
+          fail(SBAMD_ERR_INVALID, "value updates need sets without empty PCFs");
+      launch_update_values(v_device, set->src.as<int64_t>(), set->records, set->val.ptr,
+                           set->in_dtype == SBAMD_F64, set->tv_f64, pick_stream(set->ctx, stream));
+    });
+  }
+
+  int sbamd_plan_create(sbamd_ctx* ctx, const sbamd_set* rows, const sbamd_set* cols, int32_t op,
+                        double p, int32_t out_kind, const sbamd_tile_shape* shape, sbamd_plan** out)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      need(rows, "rows");
+      need(out, "out");
+      *out = nullptr;
+      if (op == SBAMD_OP_LP && !(p >= 1.0))
+        fail(SBAMD_ERR_INVALID, "p must be >= 1.");
+      (void)kernel_op(op, p);
+      auto pl = std::make_unique<sbamd_plan>();
+      pl->ctx = ctx;
+      pl->op = op;
+      pl->p = op == SBAMD_OP_LP ? p : 1.0;
+      pl->out_kind = out_kind;
+      pl->precision = rows->precision;
+      pl->tt_f64 = rows->tt_f64;
+      pl->tv_f64 = rows->tv_f64;
+      pl->sets.push_back(rows);
+      if (cols)
+      {
+        if (out_kind != SBAMD_OUT_DENSE)
+          fail(SBAMD_ERR_INVALID, "cross plans (cols != NULL) need SBAMD_OUT_DENSE");
+        if (cols->precision != rows->precision || cols->tt_f64 != rows->tt_f64 ||
+            cols->tv_f64 != rows->tv_f64)
+          fail(SBAMD_ERR_INVALID, "row and column sets must share dtype and precision");
+        pl->cdist = true;
+        pl->sets.push_back(cols);
+        pl->pairs = static_cast<uint64_t>(rows->n) * static_cast<uint64_t>(cols->n);
+        pl->out_count = pl->pairs;
+        pl->segments = static_cast<uint64_t>(cols->n) * rows->records +
+                       static_cast<uint64_t>(rows->n) * cols->records - pl->pairs;
+      }
+      else
+      {
+        const uint64_t n = static_cast<uint64_t>(rows->n);
+        const uint64_t M = static_cast<uint64_t>(rows->records);
+        if (out_kind == SBAMD_OUT_DISTMAT)
+        {
+          pl->pairs = n > 1 ? tri(n) : 0;
+          pl->segments = n > 1 ? (n - 1) * M - pl->pairs : 0;
+        }
+        else if (out_kind == SBAMD_OUT_SYMMAT)
+        {
+          pl->pairs = n * (n + 1) / 2;
+          pl->segments = n > 0 ? n * M - (n > 1 ? tri(n) : 0) : 0;
+        }
+        else
+          fail(SBAMD_ERR_INVALID, "pdist / kernel plans need SBAMD_OUT_DISTMAT or SBAMD_OUT_SYMMAT");
+        pl->out_count = pl->pairs;
+      }
+      Problem pr{0, 0, 0, rows->n};
+      pl->h_probs.push_back(pr);
+      pl->input_bytes = 0;
+      for (const Set* s : pl->sets)
+        pl->input_bytes += s->tend.bytes + s->val.bytes + s->roff.bytes;
+      if (shape)
+      {
+        if (shape->rows)
+          pl->R = shape->rows;
+        if (shape->cols)
+          pl->C = shape->cols;
+        if (shape->cap)
+          pl->cap = shape->cap;
+      }
+      if (!pl->cdist)
+        pl->C = pl->R;
+      check_shape(pl->R, pl->C, pl->cap);
+      pl->rebuild_tiles();
+      *out = pl.release();
+    });
+  }
+
+  int sbamd_plan_create_batched(sbamd_ctx* ctx, int32_t k, const sbamd_set* const* sets,
+                                int32_t op, double p, int32_t out_kind,
+                                const sbamd_tile_shape* shape, sbamd_plan** out)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      need(sets, "sets");
+      need(out, "out");
+      *out = nullptr;
+      if (k < 1 || k > 65535)
+        fail(SBAMD_ERR_INVALID, "batch size must be in [1, 65535]");
+      if (out_kind != SBAMD_OUT_DISTMAT && out_kind != SBAMD_OUT_SYMMAT)
+        fail(SBAMD_ERR_INVALID, "batched plans are pdist / kernel (DISTMAT or SYMMAT)");
+      if (op == SBAMD_OP_LP && !(p >= 1.0))
+        fail(SBAMD_ERR_INVALID, "p must be >= 1.");
+      (void)kernel_op(op, p);
+      auto pl = std::make_unique<sbamd_plan>();
+      pl->ctx = ctx;
+      pl->batched = true;
+      pl->op = op;
+      pl->p = op == SBAMD_OP_LP ? p : 1.0;
+      pl->out_kind = out_kind;
+      need(sets[0], "sets[0]");
+      pl->precision = sets[0]->precision;
+      pl->tt_f64 = sets[0]->tt_f64;
+      pl->tv_f64 = sets[0]->tv_f64;
+      int64_t pcf_base = 0, rec_base = 0;
+      uint64_t out_base = 0;
+      std::vector<int64_t> roff_all{0};
+      std::vector<int64_t> perm_all;
+      for (int32_t b = 0; b < k; ++b)
+      {
+        const Set* s = sets[b];
+        need(s, "sets[b]");
+        if (s->precision != pl->precision || s->tt_f64 != pl->tt_f64 || s->tv_f64 != pl->tv_f64)
+          fail(SBAMD_ERR_INVALID, "all sets of a batch must share dtype and precision");
+        pl->sets.push_back(s);
+        Problem pr{pcf_base, pcf_base, static_cast<int64_t>(out_base), s->n};
+        pl->h_probs.push_back(pr);
+        const uint64_t n = static_cast<uint64_t>(s->n);
+        const uint64_t M = static_cast<uint64_t>(s->records);
+        uint64_t pairs = out_kind == SBAMD_OUT_DISTMAT ? (n > 1 ? tri(n) : 0) : n * (n + 1) / 2;
+        pl->pairs += pairs;
+        if (out_kind == SBAMD_OUT_DISTMAT)
+          pl->segments += n > 1 ? (n - 1) * M - pairs : 0;
+        else
+          pl->segments += n > 0 ? n * M - (n > 1 ? tri(n) : 0) : 0;
+        out_base += pairs;
+        for (int64_t q = 0; q < s->n; ++q)
+        {
+          roff_all.push_back(rec_base + s->h_roff[q + 1]);
+          perm_all.push_back(s->h_perm[q]);
+        }
+        pcf_base += s->n;
+        rec_base += s->records;
+      }
+      pl->out_count = out_base;
+      use_device(ctx);
+      const size_t tb = pl->tt_f64 ? 8 : 4, vb = pl->tv_f64 ? 8 : 4;
+      pl->b_tend = DeviceBuffer(static_cast<size_t>(rec_base) * tb);
+      pl->b_val = DeviceBuffer(static_cast<size_t>(rec_base) * vb);
+      int64_t at = 0;
+      for (int32_t b = 0; b < k; ++b)
+      {
+        const Set* s = sets[b];
+        if (s->records == 0)
+          continue;
+        SBAMD_HIP(hipMemcpy(static_cast<char*>(pl->b_tend.ptr) + at * tb, s->tend.ptr,
+                            static_cast<size_t>(s->records) * tb, hipMemcpyDeviceToDevice));
+        SBAMD_HIP(hipMemcpy(static_cast<char*>(pl->b_val.ptr) + at * vb, s->val.ptr,
+                            static_cast<size_t>(s->records) * vb, hipMemcpyDeviceToDevice));
+        at += s->records;
+      }
+      upload(pl->b_roff, roff_all);
+      upload(pl->b_perm, perm_all);
+      pl->input_bytes = pl->b_tend.bytes + pl->b_val.bytes + pl->b_roff.bytes;
+      if (shape)
+      {
+        if (shape->rows)
+          pl->R = shape->rows;
+        if (shape->cap)
+          pl->cap = shape->cap;
+      }
+      pl->C = pl->R;
+      check_shape(pl->R, pl->C, pl->cap);
+      pl->rebuild_tiles();
+      *out = pl.release();
+    });
+  }
+
+  int sbamd_plan_destroy(sbamd_plan* plan)
+  {
+    return guarded([&] {
+      if (!plan)
+        return;
+      use_device(plan->ctx);
+      delete plan;
+    });
+  }
+
+  int sbamd_plan_info(const sbamd_plan* plan, sbamd_plan_stats* info)
+  {
+    return guarded([&] {
+      need(plan, "plan");
+      need(info, "info");
+      info->pairs = plan->pairs;
+      info->segments = plan->segments;
+      info->out_count = plan->out_count;
+      info->tiles = plan->h_tiles.size();
+      info->input_bytes = plan->input_bytes;
+      info->rows = plan->R;
+      info->cols = plan->C;
+      info->cap = plan->cap;
+      info->problems = static_cast<int32_t>(plan->h_probs.size());
+    });
+  }
+
+  int sbamd_plan_out_offset(const sbamd_plan* plan, int32_t problem, uint64_t* offset)
+  {
+    return guarded([&] {
+      need(plan, "plan");
+      need(offset, "offset");
+      if (problem < 0 || problem >= static_cast<int32_t>(plan->h_probs.size()))
+        fail(SBAMD_ERR_INVALID, "problem index out of range");
+      *offset = static_cast<uint64_t>(plan->h_probs[problem].out_base);
+    });
+  }
+
+  int sbamd_plan_set_fold_threshold(sbamd_plan* plan, int64_t segments)
+  {
+    return guarded([&] {
+      need(plan, "plan");
+      if (segments < 1)
+        fail(SBAMD_ERR_INVALID, "fold threshold must be >= 1");
+      plan->fold_threshold = segments;
+      plan->rebuild_tiles();
+    });
+  }
+
+  int sbamd_plan_set_shape(sbamd_plan* plan, const sbamd_tile_shape* shape)
+  {
+    return guarded([&] {
+      need(plan, "plan");
+      need(shape, "shape");
+      int R = shape->rows ? shape->rows : plan->R;
+      int C = shape->cols ? shape->cols : plan->C;
+      int cap = shape->cap ? shape->cap : plan->cap;
+      if (!plan->cdist)
+        C = R;
+      check_shape(R, C, cap);
+      plan->R = R;
+      plan->C = C;
+      plan->cap = cap;
+      plan->rebuild_tiles();
+    });
+  }
+
+  int sbamd_plan_execute(sbamd_plan* plan, void* out_device, int32_t out_dtype, void* row_sum_device,
+                         void* stream)
+  {
+    return guarded([&] {
+      need(plan, "plan");
+      check_out_dtype(out_dtype);
+      if (plan->out_count > 0)
+        need(out_device, "out_device");
+      if (row_sum_device && !plan->cdist)
+        fail(SBAMD_ERR_INVALID, "row sums are available for cross (dense) plans only");
+      use_device(plan->ctx);
+      hipStream_t st = pick_stream(plan->ctx, stream);
+      if (row_sum_device)
+        launch_fill_zero(row_sum_device, static_cast<size_t>(plan->sets[0]->n) * sizeof(double), st);
+      execute_range(*plan, out_device, out_dtype, static_cast<double*>(row_sum_device), 0,
+                    static_cast<int64_t>(plan->h_tiles.size()), st);
+    });
+  }
+
+  int sbamd_plan_backward(sbamd_plan* plan, const void* grad_out_device, const void* fwd_out_device,
+                          int32_t out_dtype, void* grad_v_device, void* grad_t_device,
+                          int32_t grad_dtype, void* stream)
+  {
+    return guarded([&] {
+      need(plan, "plan");
+      need(grad_v_device, "grad_v_device");
+      need(grad_t_device, "grad_t_device");
+      check_out_dtype(out_dtype);
+      check_out_dtype(grad_dtype);
+      if (plan->cdist || plan->batched)
+        fail(SBAMD_ERR_INVALID, "backward supports single-set pdist / l2_kernel plans");
+      const int kop = kernel_op(plan->op, plan->p);
+      if (!((kop == OPK_L2 && plan->out_kind == SBAMD_OUT_DISTMAT) ||
+            (kop == OPK_DOT && plan->out_kind == SBAMD_OUT_SYMMAT)))
+        fail(SBAMD_ERR_INVALID, "backward supports L2 pdist (p = 2) and the L2 kernel");
+      if (!plan->tt_f64)
+        fail(SBAMD_ERR_INVALID, "backward needs a float64 PCF set");
+      if (plan->out_count > 0)
+      {
+        need(grad_out_device, "grad_out_device");
+        if (kop == OPK_L2)
+          need(fwd_out_device, "fwd_out_device");
+      }
+      use_device(plan->ctx);
+      hipStream_t st = pick_stream(plan->ctx, stream);
+      const Set* s = plan->sets[0];
+      const size_t gb = grad_dtype == SBAMD_F64 ? 8 : 4;
+      launch_fill_zero(grad_v_device, static_cast<size_t>(s->points) * gb, st);
+      launch_fill_zero(grad_t_device, static_cast<size_t>(s->points) * gb, st);
+      BwdParams B;
+      std::memset(&B, 0, sizeof(B));
+      B.f = fwd_params(*plan, nullptr, out_dtype, nullptr);
+      // the gradient slots double the LDS per staged record: shrink the phase
+      while (B.f.cap > 2 && bwd_lds_bytes(plan->tv_f64, B.f.R, B.f.C, B.f.cap) > plan->ctx->lds_per_block)
+        B.f.cap /= 2;
+      B.grad_out = grad_out_device;
+      B.fwd_out = fwd_out_device;
+      B.r_src = s->src.as<int64_t>();
+      B.grad_v = grad_v_device;
+      B.grad_t = grad_t_device;
+      B.grad_f64 = grad_dtype == SBAMD_F64 ? 1 : 0;
+      B.in_f64 = out_dtype == SBAMD_F64 ? 1 : 0;
+      B.val_f64 = plan->tv_f64;
+      for (int64_t r : s->h_src)
+        if (r < 0)
+          fail(SBAMD_ERR_INVALID, "backward needs sets without empty PCFs");
+      launch_backward(B, kop, st);
+    });
+  }
+
+  int sbamd_graph_create(sbamd_plan* plan, void* out_device, int32_t out_dtype, sbamd_graph** out)
+  {
+    return guarded([&] {
+      need(plan, "plan");
+      need(out, "out");
+      *out = nullptr;
+      check_out_dtype(out_dtype);
+      use_device(plan->ctx);
+      auto g = std::make_unique<sbamd_graph>();
+      g->ctx = plan->ctx;
+      hipStream_t cs;
+      SBAMD_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+      hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess)
+      {
+        try
+        {
+          execute_range(*plan, out_device, out_dtype, nullptr, 0,
+                        static_cast<int64_t>(plan->h_tiles.size()), cs);
+        }
+        catch (...)
+        {
+          hipGraph_t tmp = nullptr;
+          (void)hipStreamEndCapture(cs, &tmp);
+          if (tmp)
+            (void)hipGraphDestroy(tmp);
+          (void)hipStreamDestroy(cs);
+          throw;
+        }
+        e = hipStreamEndCapture(cs, &g->graph);
+      }
+      (void)hipStreamDestroy(cs);
+      SBAMD_HIP(e);
+      SBAMD_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+      *out = g.release();
+    });
+  }
+
+  int sbamd_graph_launch(sbamd_graph* graph, void* stream)
+  {
+    return guarded([&] {
+      need(graph, "graph");
+      use_device(graph->ctx);
+      SBAMD_HIP(hipGraphLaunch(graph->exec, pick_stream(graph->ctx, stream)));
+    });
+  }
+
+  int sbamd_graph_destroy(sbamd_graph* graph)
+  {
+    return guarded([&] {
+      if (!graph)
+        return;
+      use_device(graph->ctx);
+      if (graph->exec)
+        (void)hipGraphExecDestroy(graph->exec);
+      if (graph->graph)
+        (void)hipGraphDestroy(graph->graph);
+      delete graph;
+    });
+  }
+
+  int sbamd_autotune(sbamd_plan* plan, void* out_device, int32_t out_dtype, int32_t reps,
+                     sbamd_tile_shape* best, double* best_ms)
+  {
+    return guarded([&] {
+      need(plan, "plan");
+      check_out_dtype(out_dtype);
+      if (plan->out_count > 0)
+        need(out_device, "out_device");
+      if (reps < 1)
+        reps = 1;
+      use_device(plan->ctx);
+      hipStream_t st = plan->ctx->stream;
+      const int R0 = plan->R, C0 = plan->C, cap0 = plan->cap;
+      struct Cand
+      {
+        int R, C, cap;
+      };
+      std::vector<Cand> cands;
+      const int sides[] = {8, 16};
+      const int caps[] = {32, 64, 128, 256};
+      const size_t budget = 64 * 1024;
+      for (int R : sides)
+        for (int C : sides)
+        {
+          if (!plan->cdist && R != C)
+            continue;
+          for (int cap : caps)
+          {
+            if (fwd_lds_bytes(plan->tt_f64, plan->tv_f64, R, C, cap) > budget)
+              continue;
+            cands.push_back({R, C, cap});
+          }
+        }
+      hipEvent_t e0, e1;
+      SBAMD_HIP(hipEventCreate(&e0));
+      SBAMD_HIP(hipEventCreate(&e1));
+      double bestv = std::numeric_limits<double>::infinity();
+      Cand bc{R0, C0, cap0};
+      try
+      {
+        for (const Cand& c : cands)
+        {
+          plan->R = c.R;
+          plan->C = c.C;
+          plan->cap = c.cap;
+          plan->rebuild_tiles();
+          execute_range(*plan, out_device, out_dtype, nullptr, 0,
+                        static_cast<int64_t>(plan->h_tiles.size()), st); // warm-up
+          SBAMD_HIP(hipEventRecord(e0, st));
+          for (int r = 0; r < reps; ++r)
+            execute_range(*plan, out_device, out_dtype, nullptr, 0,
+                          static_cast<int64_t>(plan->h_tiles.size()), st);
+          SBAMD_HIP(hipEventRecord(e1, st));
+          SBAMD_HIP(hipEventSynchronize(e1));
+          float ms = 0.f;
+          SBAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+          const double per = static_cast<double>(ms) / reps;
+          if (per < bestv)
+          {
+            bestv = per;
+            bc = c;
+          }
+        }
+      }
+      catch (...)
+      {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      plan->R = bc.R;
+      plan->C = bc.C;
+      plan->cap = bc.cap;
+      plan->rebuild_tiles();
+      if (best)
+      {
+        best->rows = bc.R;
+        best->cols = bc.C;
+        best->cap = bc.cap;
+        best->reserved = 0;
+      }
+      if (best_ms)
+        *best_ms = bestv;
+    });
+  }
+
+  int sbamd_pairwise_submit(sbamd_ctx* ctx, const sbamd_pcf_set* rows, const sbamd_pcf_set* cols,
+                            int32_t op, double p, int32_t out_kind, int32_t precision, void* out_host,
+                            int32_t out_dtype, sbamd_job** out)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      need(rows, "rows");
+      need(out, "out");
+      *out = nullptr;
+      check_out_dtype(out_dtype);
+      if (op == SBAMD_OP_LP && !(p >= 1.0))
+        fail(SBAMD_ERR_INVALID, "p must be >= 1.");
+      (void)kernel_op(op, p);
+      // Validate on the caller's thread so input errors surface at submit.
+      validate_set(*rows);
+      if (cols)
+        validate_set(*cols);
+      if (rows->n > 0 || (cols && cols->n > 0))
+        need(out_host, "out_host");
+      auto job = std::make_unique<sbamd_job>();
+      sbamd_job* jp = job.get();
+      const sbamd_pcf_set rcopy = *rows;
+      const bool has_cols = cols != nullptr;
+      const sbamd_pcf_set ccopy = cols ? *cols : sbamd_pcf_set{};
+      uint64_t total = 0;
+      if (has_cols)
+        total = static_cast<uint64_t>(rows->n) * static_cast<uint64_t>(cols->n);
+      else if (out_kind == SBAMD_OUT_SYMMAT)
+        total = static_cast<uint64_t>(rows->n) * (rows->n + 1) / 2;
+      else
+        total = rows->n > 1 ? tri(static_cast<uint64_t>(rows->n)) : 0;
+      jp->total = total;
+      jp->worker = std::thread([=]() {
+        int status = SBAMD_OK;
+        std::string err;
+        try
+        {
+          SBAMD_HIP(hipSetDevice(ctx->device));
+          if (total > 0)
+          {
+            sbamd_set rs, cs;
+            make_set(ctx, rcopy, precision, rs);
+            if (has_cols)
+              make_set(ctx, ccopy, precision, cs);
+            sbamd_plan* pl = nullptr;
+            int rc = sbamd_plan_create(ctx, &rs, has_cols ? &cs : nullptr, op, p, out_kind, nullptr, &pl);
+            if (rc != SBAMD_OK)
+              fail(rc, g_last_error);
+            std::unique_ptr<sbamd_plan> plan(pl);
+            const size_t eb = out_dtype == SBAMD_F64 ? 8 : 4;
+            DeviceBuffer dout(static_cast<size_t>(plan->out_count) * eb);
+            hipStream_t st;
+            SBAMD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            std::unique_ptr<std::remove_pointer<hipStream_t>::type, decltype(&hipStreamDestroy)> sguard(st, &hipStreamDestroy);
+            const int64_t nt = static_cast<int64_t>(plan->h_tiles.size());
+            const int64_t chunks = std::min<int64_t>(nt, 64);
+            const bool diag = out_kind == SBAMD_OUT_SYMMAT;
+            hipEvent_t ev;
+            SBAMD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            std::unique_ptr<std::remove_pointer<hipEvent_t>::type, decltype(&hipEventDestroy)> eguard(ev, &hipEventDestroy);
+            for (int64_t c = 0; c < chunks; ++c)
+            {
+              if (jp->cancel.load())
+                fail(SBAMD_ERR_CANCELLED, "cancelled");
+              const int64_t t0 = nt * c / chunks, t1 = nt * (c + 1) / chunks;
+              execute_range(*plan, dout.ptr, out_dtype, nullptr, t0, t1, st);
+              SBAMD_HIP(hipEventRecord(ev, st));
+              while (hipEventQuery(ev) == hipErrorNotReady)
+              {
+                if (jp->cancel.load())
+                  break;
+                std::this_thread::sleep_for(std::chrono::microseconds(200));
+              }
+              SBAMD_HIP(hipStreamSynchronize(st));
+              uint64_t done = 0;
+              for (int64_t k = t0; k < t1; ++k)
+                done += tile_pairs(plan->h_tiles[k], diag);
+              jp->progress += done;
+            }
+            SBAMD_HIP(hipMemcpy(out_host, dout.ptr, static_cast<size_t>(plan->out_count) * eb,
+                                hipMemcpyDeviceToHost));
+          }
+        }
+        catch (const Error& e)
+        {
+          status = e.code;
+          err = e.msg;
+        }
+        catch (const std::exception& e)
+        {
+          status = SBAMD_ERR_RUNTIME;
+          err = e.what();
+        }
+        std::lock_guard<std::mutex> lk(jp->mu);
+        jp->status = status;
+        jp->error = err;
+        jp->done = true;
+        jp->cv.notify_all();
+      });
+      *out = job.release();
+    });
+  }
+
+  int sbamd_job_wait(sbamd_job* job, int64_t timeout_ms)
+  {
+    if (!job)
+    {
+      g_last_error = "job is NULL";
+      return SBAMD_ERR_INVALID;
+    }
+    std::unique_lock<std::mutex> lk(job->mu);
+    if (timeout_ms < 0)
+      job->cv.wait(lk, [&] { return job->done; });
+    else if (!job->cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return job->done; }))
+      return SBAMD_TIMEOUT;
+    if (job->status != SBAMD_OK)
+      g_last_error = job->error;
+    return job->status;
+  }
+
+  int sbamd_job_progress(sbamd_job* job, uint64_t* done, uint64_t* total)
+  {
+    return guarded([&] {
+      need(job, "job");
+      if (done)
+        *done = job->progress.load();
+      if (total)
+        *total = job->total.load();
+    });
+  }
+
+  int sbamd_job_cancel(sbamd_job* job)
+  {
+    return guarded([&] {
+      need(job, "job");
+      job->cancel = true;
+    });
+  }
+
+  int sbamd_job_destroy(sbamd_job* job)
+  {
+    return guarded([&] {
+      if (!job)
+        return;
+      job->cancel = true;
+      if (job->worker.joinable())
+        job->worker.join();
+      delete job;
+    });
+  }
+
+  int sbamd_lp_distance(sbamd_ctx* ctx, const sbamd_pcf_set* f, const sbamd_pcf_set* g, double p,
+                        double* out)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      need(f, "f");
+      need(g, "g");
+      need(out, "out");
+      if (f->n != 1 || g->n != 1)
+        fail(SBAMD_ERR_INVALID, "lp_distance takes one PCF on each side");
+      if (f->dtype != g->dtype)
+        fail(SBAMD_ERR_INVALID, "f and g must have the same dtype");
+      if (!(p >= 1.0))
+        fail(SBAMD_ERR_INVALID, "p must be >= 1.");
+      const int32_t prec = f->dtype == SBAMD_F64 ? SBAMD_PREC_FP64 : SBAMD_PREC_FP32C;
+      sbamd_set fs, gs;
+      make_set(ctx, *f, prec, fs);
+      make_set(ctx, *g, prec, gs);
+      sbamd_plan* pl = nullptr;
+      const int32_t op = p == 1.0 ? SBAMD_OP_L1 : SBAMD_OP_LP;
+      int rc = sbamd_plan_create(ctx, &fs, &gs, op, p, SBAMD_OUT_DENSE, nullptr, &pl);
+      if (rc != SBAMD_OK)
+        fail(rc, g_last_error);
+      std::unique_ptr<sbamd_plan> plan(pl);
+      DeviceBuffer dout(sizeof(double));
+      execute_range(*plan, dout.ptr, SBAMD_F64, nullptr, 0, static_cast<int64_t>(plan->h_tiles.size()),
+                    ctx->stream);
+      SBAMD_HIP(hipStreamSynchronize(ctx->stream));
+      double r = 0.0;
+      SBAMD_HIP(hipMemcpy(&r, dout.ptr, sizeof(double), hipMemcpyDeviceToHost));
+      *out = r;
+    });
+  }
+
+  int sbamd_iterate_rectangles(sbamd_ctx* ctx, const sbamd_pcf_set* f, const sbamd_pcf_set* g,
+                               double a, double b, void* out, int64_t cap, int64_t* count)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      need(f, "f");
+      need(g, "g");
+      need(count, "count");
+      if (f->n != 1 || g->n != 1)
+        fail(SBAMD_ERR_INVALID, "iterate_rectangles takes one PCF on each side");
+      if (f->dtype != g->dtype)
+        fail(SBAMD_ERR_INVALID, "f and g must have the same dtype");
+      validate_set(*f);
+      validate_set(*g);
+      if (cap > 0)
+        need(out, "out");
+      use_device(ctx);
+      const size_t eb = f->dtype == SBAMD_F64 ? 8 : 4;
+      // Empty PCFs are the zero PCF [(0, 0)].
+      const double zero64 = 0.0;
+      const float zero32 = 0.f;
+      auto pts = [&](const sbamd_pcf_set* s, const void*& t, const void*& v) {
+        int64_t m = s->offsets[1] - s->offsets[0];
+        if (m == 0)
+        {
+          t = f->dtype == SBAMD_F64 ? static_cast<const void*>(&zero64) : static_cast<const void*>(&zero32);
+          v = t;
+          return int64_t(1);
+        }
+        t = static_cast<const char*>(s->t) + s->offsets[0] * eb;
+        v = static_cast<const char*>(s->v) + s->offsets[0] * eb;
+        return m;
+      };
+      const void *ft, *fv, *gt, *gv;
+      const int64_t mf = pts(f, ft, fv);
+      const int64_t mg = pts(g, gt, gv);
+      DeviceBuffer dft(mf * eb), dfv(mf * eb), dgt(mg * eb), dgv(mg * eb);
+      DeviceBuffer dout(static_cast<size_t>(std::max<int64_t>(cap, 1)) * 4 * eb), dcnt(sizeof(int64_t));
+      SBAMD_HIP(hipMemcpy(dft.ptr, ft, mf * eb, hipMemcpyHostToDevice));
+      SBAMD_HIP(hipMemcpy(dfv.ptr, fv, mf * eb, hipMemcpyHostToDevice));
+      SBAMD_HIP(hipMemcpy(dgt.ptr, gt, mg * eb, hipMemcpyHostToDevice));
+      SBAMD_HIP(hipMemcpy(dgv.ptr, gv, mg * eb, hipMemcpyHostToDevice));
+      if (f->dtype == SBAMD_F64)
+        launch_rectangles(dft.as<double>(), dfv.as<double>(), mf, dgt.as<double>(), dgv.as<double>(),
+                          mg, a, b, dout.as<double>(), cap, dcnt.as<int64_t>(), ctx->stream);
+      else
+        launch_rectangles_f32(dft.as<float>(), dfv.as<float>(), mf, dgt.as<float>(), dgv.as<float>(),
+                              mg, static_cast<float>(a), static_cast<float>(b), dout.as<float>(), cap,
+                              dcnt.as<int64_t>(), ctx->stream);
+      SBAMD_HIP(hipStreamSynchronize(ctx->stream));
+      int64_t n = 0;
+      SBAMD_HIP(hipMemcpy(&n, dcnt.ptr, sizeof(int64_t), hipMemcpyDeviceToHost));
+      *count = n;
+      const int64_t w = std::min(n, cap);
+      if (w > 0)
+        SBAMD_HIP(hipMemcpy(out, dout.ptr, static_cast<size_t>(w) * 4 * eb, hipMemcpyDeviceToHost));
+    });
+  }
+
+  int sbamd_count_segments(const sbamd_pcf_set* rows, const sbamd_pcf_set* cols,
+                           int32_t include_diagonal, uint64_t* segments)
+  {
+    return guarded([&] {
+      need(rows, "rows");
+      need(segments, "segments");
+      validate_set(*rows);
+      auto mr = record_counts(*rows);
+      uint64_t M = 0;
+      for (int64_t x : mr)
+        M += static_cast<uint64_t>(x);
+      const uint64_t n = static_cast<uint64_t>(rows->n);
+      if (cols)
+      {
+        validate_set(*cols);
+        auto mc = record_counts(*cols);
+        uint64_t M2 = 0;
+        for (int64_t x : mc)
+          M2 += static_cast<uint64_t>(x);
+        const uint64_t n2 = static_cast<uint64_t>(cols->n);
+        *segments = n2 * M + n * M2 - n * n2;
+      }
+      else if (include_diagonal)
+        *segments = n > 0 ? n * M - (n > 1 ? tri(n) : 0) : 0;
+      else
+        *segments = n > 1 ? (n - 1) * M - tri(n) : 0;
+    });
+  }
+
+  int sbamd_plan_tiles_host(int64_t n_rows, const int64_t* m_rows, int64_t n_cols,
+                            const int64_t* m_cols, int32_t tile_rows, int32_t tile_cols,
+                            int32_t* tiles, int64_t cap, int64_t* n_tiles, int64_t* perm_rows)
+  {
+    return guarded([&] {
+      need(n_tiles, "n_tiles");
+      if (n_rows < 0 || (n_rows > 0 && !m_rows))
+        fail(SBAMD_ERR_INVALID, "bad row counts");
+      std::vector<int64_t> mr(m_rows, m_rows + n_rows);
+      std::vector<int64_t> pr;
+      sort_by_records(mr, pr);
+      std::vector<int64_t> mrs(mr.size());
+      for (size_t q = 0; q < pr.size(); ++q)
+        mrs[q] = mr[pr[q]];
+      std::vector<Tile> out;
+      if (m_cols)
+      {
+        std::vector<int64_t> mc(m_cols, m_cols + n_cols);
+        std::vector<int64_t> pc;
+        sort_by_records(mc, pc);
+        std::vector<int64_t> mcs(mc.size());
+        for (size_t q = 0; q < pc.size(); ++q)
+          mcs[q] = mc[pc[q]];
+        check_shape(tile_rows, tile_cols, 64);
+        build_tiles(mrs, &mcs, tile_rows, tile_cols, false, 0, std::numeric_limits<int64_t>::max(), out);
+      }
+      else
+      {
+        check_shape(tile_rows, tile_rows, 64);
+        build_tiles(mrs, nullptr, tile_rows, tile_rows, false, 0, std::numeric_limits<int64_t>::max(), out);
+      }
+      *n_tiles = static_cast<int64_t>(out.size());
+      if (tiles)
+        for (int64_t k = 0; k < std::min<int64_t>(cap, static_cast<int64_t>(out.size())); ++k)
+        {
+          tiles[4 * k + 0] = out[k].r0;
+          tiles[4 * k + 1] = out[k].c0;
+          tiles[4 * k + 2] = out[k].nr;
+          tiles[4 * k + 3] = out[k].nc;
+        }
+      if (perm_rows)
+        for (size_t q = 0; q < pr.size(); ++q)
+          perm_rows[q] = pr[q];
+    });
+  }
+}

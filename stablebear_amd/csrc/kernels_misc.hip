@@ -1,0 +1,153 @@
+// Small kernels: the general bounded walk that lists rectangles, value
+// refresh of a staged set, and a zero fill (used instead of hipMemsetAsync so
+// that captured graphs replay correctly).
+#include "walk_common.hpp"
+
+namespace sbamd
+{
+  // Last index with t <= a, clamped to 0 (iterate_rectangles.hpp:23-35).
+  template <typename T>
+  __device__ int64_t start_index(const T* t, int64_t m, T a)
+  {
+    int64_t lo = 0, hi = m;
+    while (lo < hi)
+    {
+      const int64_t mid = lo + (hi - lo) / 2;
+      if (a < t[mid])
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    return lo == 0 ? 0 : lo - 1;
+  }
+
+  // The bounded walk of iterate_rectangles.hpp:37-104 on raw points (not the
+  // dropped-duplicate records), one thread: it reproduces every rectangle the
+  // reference emits, zero-length ones included.
+  template <typename T>
+  __global__ void rectangles_kernel(const T* ft, const T* fvals, int64_t mf, const T* gt,
+                                    const T* gvals, int64_t mg, T a, T b, T* out, int64_t cap,
+                                    int64_t* count)
+  {
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+      return;
+    int64_t fi = start_index(ft, mf, a);
+    int64_t gi = start_index(gt, mg, a);
+    T t = a;
+    T tprev = a;
+    int64_t n = 0;
+    while (t < b)
+    {
+      tprev = t;
+      const T fv = fvals[fi];
+      const T gv = gvals[gi];
+      const bool fnext = fi + 1 < mf;
+      const bool gnext = gi + 1 < mg;
+      bool done = false;
+      if (fnext && gnext)
+      {
+        const T delta = ft[fi + 1] - gt[gi + 1];
+        if (delta <= 0)
+          ++fi;
+        if (delta >= 0)
+          ++gi;
+      }
+      else if (fnext)
+        ++fi;
+      else if (gnext)
+        ++gi;
+      else
+        done = true;
+      T r;
+      if (done)
+        r = b;
+      else
+      {
+        const T mx = ft[fi] > gt[gi] ? ft[fi] : gt[gi];
+        r = mx < b ? mx : b;
+        t = r;
+      }
+      if (n < cap)
+      {
+        out[4 * n + 0] = tprev;
+        out[4 * n + 1] = r;
+        out[4 * n + 2] = fv;
+        out[4 * n + 3] = gv;
+      }
+      ++n;
+      if (done)
+        break;
+    }
+    *count = n;
+  }
+
+  void launch_rectangles(const double* ft, const double* fv, int64_t mf, const double* gt,
+                         const double* gv, int64_t mg, double a, double b, double* out,
+                         int64_t cap, int64_t* count, hipStream_t stream)
+  {
+    hipLaunchKernelGGL((rectangles_kernel<double>), dim3(1), dim3(64), 0, stream, ft, fv, mf, gt, gv,
+                       mg, a, b, out, cap, count);
+    SBAMD_HIP(hipGetLastError());
+  }
+
+  void launch_rectangles_f32(const float* ft, const float* fv, int64_t mf, const float* gt,
+                             const float* gv, int64_t mg, float a, float b, float* out,
+                             int64_t cap, int64_t* count, hipStream_t stream)
+  {
+    hipLaunchKernelGGL((rectangles_kernel<float>), dim3(1), dim3(64), 0, stream, ft, fv, mf, gt, gv,
+                       mg, a, b, out, cap, count);
+    SBAMD_HIP(hipGetLastError());
+  }
+
+  template <typename TI, typename TO>
+  __global__ void gather_values_kernel(const TI* __restrict__ v, const int64_t* __restrict__ src,
+                                       int64_t records, TO* __restrict__ rec)
+  {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < records)
+      rec[i] = static_cast<TO>(v[src[i]]);
+  }
+
+  void launch_update_values(const void* v, const int64_t* src, int64_t records, void* rec_val,
+                            int in_f64, int out_f64, hipStream_t stream)
+  {
+    if (records <= 0)
+      return;
+    const unsigned grid = static_cast<unsigned>((records + 255) / 256);
+    if (in_f64 && out_f64)
+      hipLaunchKernelGGL((gather_values_kernel<double, double>), dim3(grid), dim3(256), 0, stream,
+                         static_cast<const double*>(v), src, records, static_cast<double*>(rec_val));
+    else if (in_f64)
+      hipLaunchKernelGGL((gather_values_kernel<double, float>), dim3(grid), dim3(256), 0, stream,
+                         static_cast<const double*>(v), src, records, static_cast<float*>(rec_val));
+    else if (out_f64)
+      hipLaunchKernelGGL((gather_values_kernel<float, double>), dim3(grid), dim3(256), 0, stream,
+                         static_cast<const float*>(v), src, records, static_cast<double*>(rec_val));
+    else
+      hipLaunchKernelGGL((gather_values_kernel<float, float>), dim3(grid), dim3(256), 0, stream,
+                         static_cast<const float*>(v), src, records, static_cast<float*>(rec_val));
+    SBAMD_HIP(hipGetLastError());
+  }
+
+  __global__ void fill_zero_kernel(uint32_t* __restrict__ p, size_t words)
+  {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < words;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+      p[i] = 0u;
+  }
+
+  void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream)
+  {
+    if (bytes == 0)
+      return;
+    if (bytes % 4 != 0)
+      fail(SBAMD_ERR_INVALID, "fill size must be a multiple of 4 bytes");
+    const size_t words = bytes / 4;
+    size_t blocks = (words + 255) / 256;
+    if (blocks > 65536)
+      blocks = 65536;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream,
+                       static_cast<uint32_t*>(ptr), words);
+    SBAMD_HIP(hipGetLastError());
+  }
+}

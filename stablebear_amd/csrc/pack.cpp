@@ -1,0 +1,211 @@
+// Host-side packing of PCF sets into the device record layout, and tile
+// planning. Replaces the reference's CSR flattening (init_pcf_data,
+// include/sbear/cuda/cuda_pcf_data_manager.cuh:13-23; OffsetDataManager::init,
+// offset_data_manager.hpp:42-68) and its block scheduler
+// (cuda_block_scheduler.hpp:29-129).
+#include "sbamd_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <numeric>
+#include <sstream>
+
+namespace sbamd
+{
+  namespace
+  {
+    template <typename T>
+    double tval(const sbamd_pcf_set& s, int64_t k)
+    {
+      return static_cast<double>(static_cast<const T*>(s.t)[k]);
+    }
+
+    double time_at(const sbamd_pcf_set& s, int64_t k)
+    {
+      return s.dtype == SBAMD_F64 ? tval<double>(s, k) : tval<float>(s, k);
+    }
+
+    double value_at(const sbamd_pcf_set& s, int64_t k)
+    {
+      return s.dtype == SBAMD_F64 ? static_cast<const double*>(s.v)[k]
+                                  : static_cast<double>(static_cast<const float*>(s.v)[k]);
+    }
+
+    void check_header(const sbamd_pcf_set& s)
+    {
+      if (s.dtype != SBAMD_F32 && s.dtype != SBAMD_F64)
+        fail(SBAMD_ERR_INVALID, "PCF set dtype must be SBAMD_F32 or SBAMD_F64");
+      if (s.n < 0)
+        fail(SBAMD_ERR_INVALID, "PCF set size must be non-negative");
+      if (s.n > std::numeric_limits<int32_t>::max() / 2)
+        fail(SBAMD_ERR_INVALID, "PCF set too large (more than 2^30 PCFs)");
+      if (s.n > 0 && !s.offsets)
+        fail(SBAMD_ERR_INVALID, "PCF set offsets are NULL");
+      if (s.n == 0)
+        return;
+      if (s.offsets[0] != 0)
+        fail(SBAMD_ERR_INVALID, "PCF set offsets must start at 0");
+      for (int64_t i = 0; i < s.n; ++i)
+        if (s.offsets[i + 1] < s.offsets[i])
+          fail(SBAMD_ERR_INVALID, "PCF set offsets must be non-decreasing");
+      if (s.offsets[s.n] > 0 && (!s.t || !s.v))
+        fail(SBAMD_ERR_INVALID, "PCF set time/value arrays are NULL");
+    }
+  }
+
+  // Breakpoint rules of the reference's Pcf constructor
+  // (src/python/pypcf_support.hpp:44-87): an empty PCF is the zero PCF;
+  // times non-decreasing; first time exactly 0. Times must also be finite
+  // (the walk's end-of-PCF sentinel is +inf).
+  void validate_set(const sbamd_pcf_set& s)
+  {
+    check_header(s);
+    for (int64_t i = 0; i < s.n; ++i)
+    {
+      const int64_t b = s.offsets[i], e = s.offsets[i + 1];
+      if (e == b)
+        continue;
+      double prev = time_at(s, b);
+      if (!std::isfinite(prev))
+        fail(SBAMD_ERR_INVALID, "Breakpoint times must be finite.");
+      for (int64_t k = b + 1; k < e; ++k)
+      {
+        const double tk = time_at(s, k);
+        if (!std::isfinite(tk))
+          fail(SBAMD_ERR_INVALID, "Breakpoint times must be finite.");
+        if (tk < prev)
+          fail(SBAMD_ERR_INVALID, "Breakpoints must be given in non-decreasing time order.");
+        prev = tk;
+      }
+      const double t0 = time_at(s, b);
+      if (t0 != 0.0)
+      {
+        std::ostringstream os;
+        os << "The first breakpoint must have time t=0 (got t=" << t0 << ").";
+        fail(SBAMD_ERR_INVALID, os.str());
+      }
+    }
+  }
+
+  // Records per PCF after dropping zero-length pieces (a point is kept when
+  // the next point has a later time, or it is the last point). Keeping the
+  // last of equal times also realises the reference's start rule "last point
+  // with t <= 0" (iterate_rectangles.hpp:23-35, 44-45).
+  std::vector<int64_t> record_counts(const sbamd_pcf_set& s)
+  {
+    std::vector<int64_t> m(static_cast<size_t>(s.n));
+    for (int64_t i = 0; i < s.n; ++i)
+    {
+      const int64_t b = s.offsets[i], e = s.offsets[i + 1];
+      if (e == b)
+      {
+        m[i] = 1;
+        continue;
+      }
+      int64_t c = 1;
+      for (int64_t k = b; k + 1 < e; ++k)
+        if (time_at(s, k) < time_at(s, k + 1))
+          ++c;
+      m[i] = c;
+    }
+    return m;
+  }
+
+  void sort_by_records(const std::vector<int64_t>& m, std::vector<int64_t>& perm)
+  {
+    perm.resize(m.size());
+    std::iota(perm.begin(), perm.end(), int64_t(0));
+    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) { return m[a] < m[b]; });
+  }
+
+  void pack_set(const sbamd_pcf_set& s, PackedSet& out)
+  {
+    validate_set(s);
+    out.n = s.n;
+    out.points = s.n > 0 ? s.offsets[s.n] : 0;
+    out.in_dtype = s.dtype;
+    std::vector<int64_t> m = record_counts(s);
+    sort_by_records(m, out.perm);
+    out.roff.assign(static_cast<size_t>(s.n) + 1, 0);
+    for (int64_t q = 0; q < s.n; ++q)
+      out.roff[q + 1] = out.roff[q] + m[out.perm[q]];
+    const int64_t R = out.roff[s.n];
+    out.src.resize(static_cast<size_t>(R));
+    out.tend64.resize(static_cast<size_t>(R));
+    out.val64.resize(static_cast<size_t>(R));
+    const double inf = std::numeric_limits<double>::infinity();
+    for (int64_t q = 0; q < s.n; ++q)
+    {
+      const int64_t i = out.perm[q];
+      const int64_t b = s.offsets[i], e = s.offsets[i + 1];
+      int64_t r = out.roff[q];
+      if (e == b)
+      {
+        out.src[r] = -1;
+        out.tend64[r] = inf;
+        out.val64[r] = 0.0;
+        continue;
+      }
+      for (int64_t k = b; k < e; ++k)
+      {
+        const bool last = k + 1 == e;
+        if (!last && !(time_at(s, k) < time_at(s, k + 1)))
+          continue;
+        out.src[r] = k;
+        out.tend64[r] = last ? inf : time_at(s, k + 1) + 0.0; // + 0.0 maps -0.0 to +0.0
+        out.val64[r] = value_at(s, k);
+        ++r;
+      }
+    }
+  }
+
+  // Lower-triangle (symmetric) or full (cdist) tile grid in sorted index
+  // space, longest tiles first: rows and columns are sorted by record count,
+  // so descending block indices put the longest walks at the head of the
+  // launch and the short ones fill the tail.
+  void build_tiles(const std::vector<int64_t>& m_rows, const std::vector<int64_t>* m_cols, int R,
+                   int C, bool include_diag, uint16_t prob, int64_t fold_threshold,
+                   std::vector<Tile>& tiles)
+  {
+    const int64_t nr = static_cast<int64_t>(m_rows.size());
+    const bool sym = m_cols == nullptr;
+    const std::vector<int64_t>& mc = sym ? m_rows : *m_cols;
+    const int64_t nc = static_cast<int64_t>(mc.size());
+    if (sym && R != C)
+      fail(SBAMD_ERR_INVALID, "symmetric plans need square tiles");
+    const int64_t nbr = (nr + R - 1) / R;
+    const int64_t nbc = (nc + C - 1) / C;
+    auto block_max = [](const std::vector<int64_t>& m, int64_t b0, int64_t cnt) {
+      int64_t mx = 0;
+      for (int64_t k = b0; k < b0 + cnt; ++k)
+        mx = std::max(mx, m[k]);
+      return mx;
+    };
+    for (int64_t bi = nbr - 1; bi >= 0; --bi)
+    {
+      const int64_t r0 = bi * R;
+      const int64_t rn = std::min<int64_t>(R, nr - r0);
+      const int64_t rmax = block_max(m_rows, r0, rn);
+      const int64_t bj_end = sym ? bi : nbc - 1;
+      for (int64_t bj = bj_end; bj >= 0; --bj)
+      {
+        const int64_t c0 = bj * C;
+        const int64_t cn = std::min<int64_t>(C, nc - c0);
+        const bool diag = sym && bi == bj;
+        if (diag && !include_diag && rn < 2)
+          continue;
+        Tile t;
+        t.r0 = static_cast<int32_t>(r0);
+        t.c0 = static_cast<int32_t>(c0);
+        t.nr = static_cast<int16_t>(rn);
+        t.nc = static_cast<int16_t>(cn);
+        t.prob = prob;
+        t.flags = diag ? TILE_DIAG : 0;
+        if (rmax + block_max(mc, c0, cn) - 1 >= fold_threshold)
+          t.flags |= TILE_FOLD;
+        tiles.push_back(t);
+      }
+    }
+  }
+}

@@ -1,0 +1,207 @@
+// Internal types shared by the host runtime and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/stablebear_amd.h"
+
+namespace sbamd
+{
+  // ---------------------------------------------------------------------
+  // Error plumbing: C++ exceptions never cross the C-ABI; api.cpp converts
+  // them into status codes + a thread-local message.
+  // ---------------------------------------------------------------------
+  struct Error
+  {
+    int code;
+    std::string msg;
+  };
+
+  [[noreturn]] void fail(int code, const std::string& msg);
+  void hip_check(hipError_t e, const char* what, const char* file, int line);
+
+#define SBAMD_HIP(call) ::sbamd::hip_check((call), #call, __FILE__, __LINE__)
+
+  // ---------------------------------------------------------------------
+  // Device layout
+  // ---------------------------------------------------------------------
+
+  // One workgroup tile: rows [r0, r0 + nr) x cols [c0, c0 + nc) in the sorted
+  // PCF order of problem `prob`.
+  struct Tile
+  {
+    int32_t r0;
+    int32_t c0;
+    int16_t nr;
+    int16_t nc;
+    uint16_t prob;
+    uint16_t flags; // bit 0: diagonal tile (rows == cols); bit 1: fold partials in FP64
+  };
+  static_assert(sizeof(Tile) == 16, "Tile must stay 16 bytes");
+
+  constexpr uint16_t TILE_DIAG = 1;
+  constexpr uint16_t TILE_FOLD = 2;
+
+  // Kernel-level pair operations (SBAMD_OP_LP with p == 2 becomes OPK_L2).
+  constexpr int OPK_L1 = 0;
+  constexpr int OPK_LP = 1;
+  constexpr int OPK_DOT = 2;
+  constexpr int OPK_L2 = 3;
+
+  // Per-problem bases into the concatenated arrays of a (batched) set.
+  struct Problem
+  {
+    int64_t row_base; // first row PCF (sorted order) of the problem in r_off / r_perm
+    int64_t col_base; // first column PCF of the problem in c_off / c_perm
+    int64_t out_base; // first output element of the problem
+    int64_t n;        // PCFs in the problem (rows)
+  };
+
+  // Kernel parameters of the forward lane-per-pair tile kernel.
+  struct FwdParams
+  {
+    const int64_t* r_off;  // record offsets of the row set (sorted PCF order), n+1 per problem base
+    const void* r_tend;    // record end times (TT)
+    const void* r_val;     // record values (TV)
+    const int64_t* r_perm; // sorted position -> original index within its problem
+    const int64_t* c_off;
+    const void* c_tend;
+    const void* c_val;
+    const int64_t* c_perm;
+    const Tile* tiles;
+    const Problem* probs;
+    void* out;
+    double* row_sum; // optional (DENSE): per original row sums
+    int64_t ntiles;
+    int64_t out_cols; // DENSE row length
+    double p;
+    int32_t out_kind;
+    int32_t out_f64;
+    int32_t R, C, cap, stride;
+    int32_t symmetric; // rows and cols are the same set (pdist / kernel)
+    int32_t include_diag;
+  };
+
+  struct BwdParams
+  {
+    FwdParams f;             // forward geometry (out unused)
+    const void* grad_out;    // upstream gradient per output element
+    const void* fwd_out;     // forward output (distances) for p = 2
+    const int64_t* r_src;    // record -> original flat point index
+    void* grad_v;
+    void* grad_t;
+    int32_t grad_f64;
+    int32_t in_f64;  // grad_out / fwd_out are FP64
+    int32_t val_f64; // record values are FP64 (else FP32); record times are FP64
+    int32_t pad;
+  };
+
+  // ---------------------------------------------------------------------
+  // Host objects behind the opaque C handles
+  // ---------------------------------------------------------------------
+  struct DeviceBuffer
+  {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t b);
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept;
+    ~DeviceBuffer();
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+  };
+
+  // Host-side packed set (validated, zero-length segments dropped, sorted).
+  struct PackedSet
+  {
+    int64_t n = 0;          // PCFs
+    int64_t points = 0;     // original points (offsets[n])
+    int32_t in_dtype = SBAMD_F64;
+    std::vector<int64_t> perm;   // sorted position -> original PCF index
+    std::vector<int64_t> roff;   // sorted record offsets (n + 1)
+    std::vector<int64_t> src;    // record -> original flat point index
+    std::vector<double> tend64;  // record end times (inf for the last record)
+    std::vector<double> val64;   // record values
+  };
+
+  void pack_set(const sbamd_pcf_set& in, PackedSet& out);
+  void validate_set(const sbamd_pcf_set& in);
+  std::vector<int64_t> record_counts(const sbamd_pcf_set& in);
+
+  struct Ctx
+  {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    size_t lds_per_block = 0;
+  };
+
+  struct Set
+  {
+    Ctx* ctx = nullptr;
+    int32_t precision = SBAMD_PREC_FP64;
+    int32_t in_dtype = SBAMD_F64;
+    int32_t tt_f64 = 1; // record times stored as FP64
+    int32_t tv_f64 = 1; // record values stored as FP64
+    int64_t n = 0;
+    int64_t records = 0;
+    int64_t points = 0;
+    std::vector<int64_t> h_roff; // sorted record offsets
+    std::vector<int64_t> h_perm;
+    std::vector<int64_t> h_src;
+    DeviceBuffer roff, tend, val, perm, src;
+    DeviceBuffer inv_src; // original point -> record (or -1), for value updates
+  };
+
+  struct Plan
+  {
+    Ctx* ctx = nullptr;
+    std::vector<const Set*> sets; // 1 (pdist / kernel), 2 (cdist: rows, cols), or k (batched)
+    bool cdist = false;
+    bool batched = false;
+    int32_t op = SBAMD_OP_L1;
+    double p = 1.0;
+    int32_t out_kind = SBAMD_OUT_DISTMAT;
+    int32_t precision = SBAMD_PREC_FP64;
+    int32_t tt_f64 = 1, tv_f64 = 1;
+    int32_t R = 16, C = 16, cap = 64;
+    int64_t fold_threshold = 8192;
+    uint64_t pairs = 0, segments = 0, out_count = 0;
+    std::vector<Problem> h_probs;
+    std::vector<Tile> h_tiles;
+    DeviceBuffer probs, tiles;
+    // batched: concatenated set arrays
+    DeviceBuffer b_roff, b_tend, b_val, b_perm;
+    uint64_t input_bytes = 0;
+    void rebuild_tiles();
+  };
+
+  // Kernel launchers (kernels_fwd.hip / kernels_bwd.hip / kernels_misc.hip)
+  size_t fwd_lds_bytes(int tt_f64, int tv_f64, int R, int C, int cap);
+  void launch_forward(const FwdParams& prm, int tt_f64, int tv_f64, int op, int kahan,
+                      hipStream_t stream);
+  void launch_backward(const BwdParams& prm, int op, hipStream_t stream);
+  size_t bwd_lds_bytes(int tv_f64, int R, int C, int cap);
+  void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream);
+  void launch_rectangles(const double* ft, const double* fv, int64_t mf, const double* gt,
+                         const double* gv, int64_t mg, double a, double b, double* out,
+                         int64_t cap, int64_t* count, hipStream_t stream);
+  void launch_rectangles_f32(const float* ft, const float* fv, int64_t mf, const float* gt,
+                             const float* gv, int64_t mg, float a, float b, float* out,
+                             int64_t cap, int64_t* count, hipStream_t stream);
+  void launch_update_values(const void* v, const int64_t* src, int64_t records, void* rec_val,
+                            int in_f64, int out_f64, hipStream_t stream);
+
+  // Planning
+  void build_tiles(const std::vector<int64_t>& m_rows_sorted, const std::vector<int64_t>* m_cols_sorted,
+                   int R, int C, bool include_diag, uint16_t prob, int64_t fold_threshold,
+                   std::vector<Tile>& tiles);
+  void sort_by_records(const std::vector<int64_t>& m, std::vector<int64_t>& perm);
+}

@@ -1,0 +1,395 @@
+"""Host runtime around the C-ABI: the default device context, CSR packing of
+PCF containers, asynchronous tasks, and device-resident sets / plans.
+
+Everything here drives ``libstablebear_amd.so``; there is no CPU compute path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from .pcf import Pcf
+from .typing import pcf32, pcf64
+
+_ctx_lock = threading.Lock()
+_contexts: dict[int, ctypes.c_void_p] = {}
+_default_device = 0
+
+
+def set_default_device(device: int) -> None:
+    global _default_device
+    _default_device = int(device)
+
+
+def context(device: int | None = None) -> ctypes.c_void_p:
+    """The process-wide context of a HIP device (created on first use)."""
+    dev = _default_device if device is None else int(device)
+    with _ctx_lock:
+        ctx = _contexts.get(dev)
+        if ctx is None:
+            h = ctypes.c_void_p()
+            L.check(L.lib().sbamd_ctx_create(dev, ctypes.byref(h)))
+            ctx = h
+            _contexts[dev] = ctx
+        return ctx
+
+
+# ---------------------------------------------------------------------------
+# CSR packing
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Csr:
+    """PCF set in CSR form; keeps the numpy arrays alive for the C structs."""
+
+    offsets: np.ndarray  # int64, n + 1
+    t: np.ndarray
+    v: np.ndarray
+    dtype: int  # L.F32 / L.F64
+
+    @property
+    def n(self) -> int:
+        return len(self.offsets) - 1
+
+    def struct(self) -> L.PcfSet:
+        s = L.PcfSet()
+        s.n = self.n
+        s.offsets = self.offsets.ctypes.data_as(L.c_i64p)
+        s.t = self.t.ctypes.data if self.t.size else None
+        s.v = self.v.ctypes.data if self.v.size else None
+        s.dtype = self.dtype
+        return s
+
+
+def csr_from_arrays(offsets, t, v) -> Csr:
+    """CSR set from offsets (n+1), times and values (float32 or float64)."""
+    t = np.asarray(t)
+    dt = np.float64 if t.dtype == np.float64 else np.float32
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    t = np.ascontiguousarray(t, dtype=dt)
+    v = np.ascontiguousarray(v, dtype=dt)
+    return Csr(offsets, t, v, L.F64 if dt == np.float64 else L.F32)
+
+
+def csr_from_pcfs(pcfs, dtype) -> Csr:
+    """CSR set from a list of Pcf of one dtype (pcf32 or pcf64)."""
+    np_dt = np.float64 if dtype is pcf64 else np.float32
+    n = len(pcfs)
+    lens = np.fromiter((p._arr.shape[0] for p in pcfs), dtype=np.int64, count=n)
+    offsets = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(lens, out=offsets[1:])
+    if n:
+        pts = np.concatenate([p._arr for p in pcfs], axis=0).astype(np_dt, copy=False)
+        t = np.ascontiguousarray(pts[:, 0])
+        v = np.ascontiguousarray(pts[:, 1])
+    else:
+        t = np.zeros(0, dtype=np_dt)
+        v = np.zeros(0, dtype=np_dt)
+    return Csr(offsets, t, v, L.F64 if np_dt == np.float64 else L.F32)
+
+
+# ---------------------------------------------------------------------------
+# Tasks (the reference's StoppableTask protocol)
+# ---------------------------------------------------------------------------
+
+class Task:
+    """Handle of an asynchronous pairwise job: wait_for / progress / stop.
+
+    Mirrors the methods ``stablebear/async_task.py`` polls
+    (``include/sbear/task.hpp:15-193``). Holds the host buffers the job reads
+    and writes until it is destroyed.
+    """
+
+    def __init__(self, job: ctypes.c_void_p | None, keepalive, desc: str, total: int):
+        self._job = job
+        self._keep = keepalive
+        self._desc = desc
+        self._total = int(total)
+        self._done = job is None
+        self._error = None
+
+    def wait_for(self, timeout_ms: int) -> bool:
+        if self._done:
+            if self._error:
+                err, self._error = self._error, None
+                raise err
+            return True
+        rc = L.lib().sbamd_job_wait(self._job, int(timeout_ms))
+        if rc == L.TIMEOUT:
+            return False
+        self._done = True
+        try:
+            L.check(rc)
+        finally:
+            self._release()
+        return True
+
+    def wait(self) -> None:
+        while not self.wait_for(1000):
+            pass
+
+    def request_stop(self) -> None:
+        if self._job is not None and not self._done:
+            L.lib().sbamd_job_cancel(self._job)
+
+    def work_total(self) -> int:
+        return self._total
+
+    def work_completed(self) -> int:
+        if self._job is None:
+            return self._total
+        done = ctypes.c_uint64()
+        tot = ctypes.c_uint64()
+        L.lib().sbamd_job_progress(self._job, ctypes.byref(done), ctypes.byref(tot))
+        return int(done.value)
+
+    def work_step(self) -> int:
+        return 0
+
+    def work_step_desc(self) -> str:
+        return self._desc
+
+    def work_step_unit(self) -> str:
+        return "integral"
+
+    def _release(self):
+        if self._job is not None:
+            L.lib().sbamd_job_destroy(self._job)
+            self._job = None
+        self._keep = None
+
+    def __del__(self):
+        try:
+            if self._job is not None:
+                L.lib().sbamd_job_cancel(self._job)
+                self._release()
+        except Exception:
+            pass
+
+
+def done_task(desc="", total=0) -> Task:
+    return Task(None, None, desc, total)
+
+
+def submit(rows: Csr, cols: Csr | None, op: int, p: float, out_kind: int, precision: int,
+           out: np.ndarray, desc: str, total: int) -> Task:
+    """Start a host-to-host pairwise job writing into ``out`` (kept alive by the task)."""
+    rs = rows.struct()
+    cs = cols.struct() if cols is not None else None
+    job = ctypes.c_void_p()
+    out_dtype = L.F64 if out.dtype == np.float64 else L.F32
+    rc = L.lib().sbamd_pairwise_submit(context(), ctypes.byref(rs),
+                                       ctypes.byref(cs) if cs is not None else None, op,
+                                       float(p), out_kind, precision,
+                                       out.ctypes.data if out.size else None, out_dtype,
+                                       ctypes.byref(job))
+    L.check(rc)
+    return Task(job, (rows, cols, rs, cs, out), desc, total)
+
+
+# ---------------------------------------------------------------------------
+# Single-pair helpers
+# ---------------------------------------------------------------------------
+
+def _one(p: Pcf) -> Csr:
+    return csr_from_pcfs([p], p.dtype)
+
+
+def lp_distance(f: Pcf, g: Pcf, p: float) -> float:
+    a, b = _one(f), _one(g)
+    sa, sb_ = a.struct(), b.struct()
+    out = ctypes.c_double()
+    L.check(L.lib().sbamd_lp_distance(context(), ctypes.byref(sa), ctypes.byref(sb_), float(p),
+                                      ctypes.byref(out)))
+    return float(out.value)
+
+
+def rectangles(f: Pcf, g: Pcf, a: float, b: float) -> np.ndarray:
+    A, B = _one(f), _one(g)
+    sa, sb_ = A.struct(), B.struct()
+    np_dt = np.float64 if f.dtype is pcf64 else np.float32
+    cap = f.size + g.size + 4
+    out = np.zeros((cap, 4), dtype=np_dt)
+    cnt = ctypes.c_int64()
+    L.check(L.lib().sbamd_iterate_rectangles(context(), ctypes.byref(sa), ctypes.byref(sb_),
+                                             float(a), float(b), out.ctypes.data, cap,
+                                             ctypes.byref(cnt)))
+    if cnt.value > cap:
+        raise RuntimeError("rectangle count exceeds the walk bound")
+    return out[: cnt.value]
+
+
+def count_segments(rows: Csr, cols: Csr | None = None, include_diagonal=False) -> int:
+    """Exact union-segment count W of a pdist / kernel / cdist (host only)."""
+    rs = rows.struct()
+    cs = cols.struct() if cols is not None else None
+    w = ctypes.c_uint64()
+    L.check(L.lib().sbamd_count_segments(ctypes.byref(rs), ctypes.byref(cs) if cs else None,
+                                         1 if include_diagonal else 0, ctypes.byref(w)))
+    return int(w.value)
+
+
+# ---------------------------------------------------------------------------
+# Device-resident sets and plans (zero-copy execution on device buffers)
+# ---------------------------------------------------------------------------
+
+PRECISIONS = {"fp64": L.PREC_FP64, "fp32c": L.PREC_FP32C}
+
+
+class DeviceSet:
+    """A PCF set validated, sorted and uploaded once for a given precision."""
+
+    def __init__(self, csr: Csr, precision: str = "fp64", device: int | None = None):
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        self.ctx = context(device)
+        self.csr = csr
+        self.precision = precision
+        s = csr.struct()
+        h = ctypes.c_void_p()
+        L.check(L.lib().sbamd_set_create(self.ctx, ctypes.byref(s), PRECISIONS[precision],
+                                         ctypes.byref(h)))
+        self.handle = h
+        n, rec, pts = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        L.check(L.lib().sbamd_set_info(h, ctypes.byref(n), ctypes.byref(rec), ctypes.byref(pts)))
+        self.n, self.records, self.points = n.value, rec.value, pts.value
+
+    def update_values(self, v_device_ptr: int, stream: int | None = None) -> None:
+        L.check(L.lib().sbamd_set_update_values(self.handle, v_device_ptr, stream))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            L.lib().sbamd_set_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+OPS = {"l1": L.OP_L1, "lp": L.OP_LP, "dot": L.OP_DOT}
+
+
+class Plan:
+    """Tile plan of one pdist / l2_kernel / cdist (or a batch of pdists)."""
+
+    def __init__(self, rows, cols=None, op="l1", p=1.0, out_kind=None, shape=None,
+                 batched=False):
+        self._sets = list(rows) if batched else [rows] + ([cols] if cols is not None else [])
+        first = self._sets[0]
+        self.ctx = first.ctx
+        self.op = op
+        self.p = float(p)
+        if out_kind is None:
+            out_kind = L.OUT_DENSE if cols is not None else (
+                L.OUT_SYMMAT if op == "dot" else L.OUT_DISTMAT)
+        self.out_kind = out_kind
+        sh = L.TileShape()
+        if shape:
+            sh.rows, sh.cols, sh.cap = shape
+        h = ctypes.c_void_p()
+        if batched:
+            arr = (ctypes.c_void_p * len(self._sets))(*[s.handle.value for s in self._sets])
+            L.check(L.lib().sbamd_plan_create_batched(self.ctx, len(self._sets), arr, OPS[op],
+                                                      self.p, out_kind, ctypes.byref(sh),
+                                                      ctypes.byref(h)))
+        else:
+            L.check(L.lib().sbamd_plan_create(self.ctx, first.handle,
+                                              cols.handle if cols is not None else None,
+                                              OPS[op], self.p, out_kind, ctypes.byref(sh),
+                                              ctypes.byref(h)))
+        self.handle = h
+        self.refresh_stats()
+
+    def refresh_stats(self):
+        st = L.PlanStats()
+        L.check(L.lib().sbamd_plan_info(self.handle, ctypes.byref(st)))
+        self.pairs = st.pairs
+        self.segments = st.segments
+        self.out_count = st.out_count
+        self.tiles = st.tiles
+        self.input_bytes = st.input_bytes
+        self.shape = (st.rows, st.cols, st.cap)
+        self.problems = st.problems
+
+    def out_offset(self, problem: int) -> int:
+        off = ctypes.c_uint64()
+        L.check(L.lib().sbamd_plan_out_offset(self.handle, problem, ctypes.byref(off)))
+        return int(off.value)
+
+    def set_fold_threshold(self, segments: int):
+        L.check(L.lib().sbamd_plan_set_fold_threshold(self.handle, int(segments)))
+
+    def set_shape(self, rows, cols, cap):
+        sh = L.TileShape(rows, cols, cap, 0)
+        L.check(L.lib().sbamd_plan_set_shape(self.handle, ctypes.byref(sh)))
+        self.refresh_stats()
+
+    def execute(self, out_ptr: int, out_f64: bool, row_sum_ptr: int | None = None,
+                stream: int | None = None):
+        L.check(L.lib().sbamd_plan_execute(self.handle, out_ptr, L.F64 if out_f64 else L.F32,
+                                           row_sum_ptr, stream))
+
+    def backward(self, grad_out_ptr, fwd_out_ptr, out_f64, grad_v_ptr, grad_t_ptr, grad_f64,
+                 stream=None):
+        L.check(L.lib().sbamd_plan_backward(self.handle, grad_out_ptr, fwd_out_ptr,
+                                            L.F64 if out_f64 else L.F32, grad_v_ptr, grad_t_ptr,
+                                            L.F64 if grad_f64 else L.F32, stream))
+
+    def autotune(self, out_ptr: int, out_f64: bool, reps: int = 3):
+        sh = L.TileShape()
+        ms = ctypes.c_double()
+        L.check(L.lib().sbamd_autotune(self.handle, out_ptr, L.F64 if out_f64 else L.F32,
+                                       int(reps), ctypes.byref(sh), ctypes.byref(ms)))
+        self.refresh_stats()
+        return (sh.rows, sh.cols, sh.cap), ms.value
+
+    def graph(self, out_ptr: int, out_f64: bool) -> "Graph":
+        return Graph(self, out_ptr, out_f64)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            L.lib().sbamd_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """hipGraph holding one captured forward launch of a plan."""
+
+    def __init__(self, plan: Plan, out_ptr: int, out_f64: bool):
+        self.plan = plan
+        h = ctypes.c_void_p()
+        L.check(L.lib().sbamd_graph_create(plan.handle, out_ptr, L.F64 if out_f64 else L.F32,
+                                           ctypes.byref(h)))
+        self.handle = h
+
+    def launch(self, stream: int | None = None):
+        L.check(L.lib().sbamd_graph_launch(self.handle, stream))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            L.lib().sbamd_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synchronize(device: int | None = None):
+    L.check(L.lib().sbamd_ctx_synchronize(context(device)))

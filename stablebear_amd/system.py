@@ -1,0 +1,93 @@
+"""Runtime knobs (``stablebear/system.py``), mapped onto this build.
+
+Every pairwise computation runs on the GPU through the HIP library, so the
+knobs that steer the reference's CPU/GPU choice are kept for source
+compatibility: ``force_cpu(True)`` is refused (there is no CPU compute path),
+``set_cuda_threshold`` / ``limit_cpus`` / ``set_parallel_eval_threshold`` are
+recorded but do not change where work runs, and ``set_block_size`` /
+``set_min_block_side`` map to the pair-tile shape of the HIP kernels.
+"""
+
+from __future__ import annotations
+
+_state = {
+    "force_cpu": False,
+    "cuda_threshold": 500,
+    "device_verbose": False,
+    "block": (16, 16),
+    "cap": 0,
+    "min_block_side": 0,
+    "cpus": 0,
+    "gpus": 0,
+    "parallel_eval_threshold": 500,
+    "precision": "fp64",
+}
+
+
+def force_cpu(on: bool):
+    if on:
+        raise RuntimeError("stablebear_amd computes pairwise integrals on the GPU only; "
+                           "force_cpu(True) is not supported")
+    _state["force_cpu"] = False
+
+
+def set_block_size(x: int, y: int):
+    """Pair-tile shape (rows x cols pairs per workgroup, rows * cols <= 256)."""
+    if x < 1 or y < 1 or x * y > 256:
+        raise ValueError("block size must satisfy 1 <= x, y and x * y <= 256")
+    _state["block"] = (int(x), int(y))
+
+
+def set_tile_cap(cap: int):
+    """Breakpoint records per PCF staged in LDS per phase (0 = default)."""
+    _state["cap"] = int(cap)
+
+
+def limit_cpus(n: int):
+    _state["cpus"] = int(n)
+
+
+def limit_gpus(n: int):
+    if n < 1:
+        raise ValueError("need at least one GPU")
+    _state["gpus"] = int(n)
+
+
+def set_cuda_threshold(n: int):
+    _state["cuda_threshold"] = int(n)
+
+
+def set_parallel_eval_threshold(n: int):
+    _state["parallel_eval_threshold"] = int(n)
+
+
+def get_parallel_eval_threshold() -> int:
+    return _state["parallel_eval_threshold"]
+
+
+def set_device_verbose(on: bool):
+    _state["device_verbose"] = bool(on)
+
+
+def set_min_block_side(n: int):
+    _state["min_block_side"] = int(n)
+
+
+def set_precision(precision: str):
+    """Default arithmetic form for float64 PCFs: "fp64" (default) or "fp32c"."""
+    if precision not in ("fp64", "fp32c"):
+        raise ValueError("precision must be 'fp64' or 'fp32c'")
+    _state["precision"] = precision
+
+
+def get_precision() -> str:
+    return _state["precision"]
+
+
+def build_type() -> str:
+    return "HIP"
+
+
+def _verbose(msg: str):
+    if _state["device_verbose"]:
+        print(msg)

@@ -1,0 +1,316 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Each oracle function is checked against (a) the reference's own known-answer
+vectors (tests/golden/reference_cases.json), (b) the reference's compiled walk
+(oracle/_ref, when built, and the vectors it wrote to tests/golden/
+ref_vectors.npz), and (c) what the mathematics fixes: exact rational
+arithmetic, the common-grid reduction to scipy's weighted Minkowski distance,
+invariants, and central finite differences for the gradient.
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from stablebear_amd import workloads
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+CASES = json.load(open(os.path.join(GOLD, "reference_cases.json")))
+REF = np.load(os.path.join(GOLD, "ref_vectors.npz"))
+
+DBL_MAX = np.finfo(np.float64).max
+
+
+def _num(x):
+    if x == "DBL_MAX":
+        return DBL_MAX
+    if x == "inf":
+        return float("inf")
+    return float(x)
+
+
+# --------------------------------------------------------------- walk ------
+
+@pytest.mark.parametrize("case", CASES["rects"], ids=lambda c: c["src"])
+def test_rects_match_reference_vectors(case):
+    got = oracle.rects(case["f"], case["g"], _num(case["a"]), _num(case["b"]))
+    exp = np.array([[_num(x) for x in r] for r in case["expected"]], dtype=np.float64)
+    assert got.shape == (len(exp), 4)
+    if len(exp):
+        np.testing.assert_array_equal(got, exp)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("case", CASES["rects"], ids=lambda c: c["src"])
+def test_rects_match_compiled_reference(case):
+    a, b = _num(case["a"]), _num(case["b"])
+    np.testing.assert_array_equal(oracle.rects(case["f"], case["g"], a, b),
+                                  oracle.rects(case["f"], case["g"], a, b, use_ref=True))
+
+
+def test_rects_swapped_arguments_mirror():
+    f, g = CASES["rects"][0]["f"], CASES["rects"][0]["g"]
+    r1 = oracle.rects(f, g, 0, 6.0)
+    r2 = oracle.rects(g, f, 0, 6.0)
+    np.testing.assert_array_equal(r1[:, :2], r2[:, :2])
+    np.testing.assert_array_equal(r1[:, 2], r2[:, 3])
+    np.testing.assert_array_equal(r1[:, 3], r2[:, 2])
+
+
+def test_rects_brute_force_union_grid():
+    """Rectangles equal both PCFs evaluated on the sorted union of breakpoints."""
+    rng = np.random.default_rng(7)
+    for _ in range(50):
+        pcfs = []
+        for m in rng.integers(1, 8, size=2):
+            t = np.concatenate([[0.0], np.sort(rng.choice(np.arange(1, 12), size=m - 1))])
+            pcfs.append(np.stack([t, rng.integers(-3, 4, size=m).astype(float)], 1))
+        f, g = pcfs
+        rect = oracle.rects(f, g, 0.0, 100.0)
+        rect = rect[rect[:, 1] > rect[:, 0]]  # drop zero-length (duplicate / tie) pieces
+        grid = np.unique(np.concatenate([f[:, 0], g[:, 0], [100.0]]))
+
+        def ev(p, x):
+            return p[np.searchsorted(p[:, 0], x, side="right") - 1, 1]
+
+        exp = np.array([[grid[k], grid[k + 1], ev(f, grid[k]), ev(g, grid[k])]
+                        for k in range(len(grid) - 1)])
+        np.testing.assert_array_equal(rect, exp)
+
+
+# ---------------------------------------------------------- integrals ------
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("case", CASES["lp_distance"], ids=lambda c: c["src"])
+def test_lp_distance_known_answers(case, dtype):
+    got = oracle.distance(case["f"], case["g"], case["p"], dtype=dtype)
+    tol = case.get("abs", 1e-5 if dtype == np.float32 else 1e-12)
+    assert got == pytest.approx(case["expected"], abs=tol)
+
+
+def _csr(pcfs):
+    off = np.zeros(len(pcfs) + 1, dtype=np.int64)
+    off[1:] = np.cumsum([len(p) for p in pcfs])
+    pts = np.concatenate([np.asarray(p, dtype=np.float64) for p in pcfs])
+    return off, pts[:, 0].copy(), pts[:, 1].copy()
+
+
+def _dense_from_pdist(c, n):
+    d = np.zeros((n, n))
+    r, cc = np.tril_indices(n, -1)
+    d[r, cc] = c
+    d[cc, r] = c
+    return d
+
+
+def _dense_from_kernel(c, n):
+    d = np.zeros((n, n))
+    r, cc = np.tril_indices(n, 0)
+    d[r, cc] = c
+    d[cc, r] = c
+    return d
+
+
+@pytest.mark.parametrize("case", CASES["pdist"], ids=lambda c: c["src"])
+def test_pdist_known_answers(case):
+    off, t, v = _csr(case["pcfs"])
+    p = case["p"]
+    op = oracle.OP_L1 if p == 1 else oracle.OP_LP
+    got = oracle.pairwise(op, p, oracle.KIND_PDIST, off, t, v)
+    np.testing.assert_allclose(_dense_from_pdist(got, len(off) - 1), case["expected_dense"],
+                               rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", CASES["l2_kernel"], ids=lambda c: c["src"])
+def test_kernel_known_answers(case):
+    off, t, v = _csr(case["pcfs"])
+    got = oracle.pairwise(oracle.OP_DOT, 1.0, oracle.KIND_KERNEL, off, t, v)
+    np.testing.assert_allclose(_dense_from_kernel(got, len(off) - 1), case["expected_dense"],
+                               rtol=1e-12, atol=1e-12)
+
+
+_REF_CASES = ["c1", "c2", "edge", "c4"]
+_REF_OPS = {"l1": (oracle.OP_L1, 1.0), "l2": (oracle.OP_LP, 2.0), "l3": (oracle.OP_LP, 3.0),
+            "dot": (oracle.OP_DOT, 1.0)}
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+@pytest.mark.parametrize("oname", list(_REF_OPS))
+@pytest.mark.parametrize("name", _REF_CASES)
+def test_oracle_bitwise_equals_reference_vectors(name, oname, dt):
+    off, t, v = REF[f"{name}_off"], REF[f"{name}_t"], REF[f"{name}_v"]
+    op, p = _REF_OPS[oname]
+    kind = oracle.KIND_KERNEL if op == oracle.OP_DOT else oracle.KIND_PDIST
+    got = oracle.pairwise(op, p, kind, off, t, v, dtype=np.float64 if dt == "f64" else np.float32)
+    np.testing.assert_array_equal(got, REF[f"{name}_{oname}_{dt}"])
+
+
+@pytest.mark.parametrize("oname", ["l1", "l2"])
+def test_oracle_cdist_bitwise_equals_reference_vectors(oname):
+    X = (REF["cx_off"], REF["cx_t"], REF["cx_v"])
+    Y = (REF["cy_off"], REF["cy_t"], REF["cy_v"])
+    op, p = _REF_OPS[oname]
+    got = oracle.pairwise(op, p, oracle.KIND_CDIST, *X, *Y)
+    np.testing.assert_array_equal(got, REF[f"cdist_{oname}_f64"])
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_oracle_bitwise_equals_compiled_reference_random(seed):
+    off, t, v = workloads.c2(80, seed=seed)
+    for op, p, kind in ((oracle.OP_L1, 1.0, 0), (oracle.OP_LP, 2.0, 0), (oracle.OP_DOT, 1.0, 1)):
+        np.testing.assert_array_equal(oracle.pairwise(op, p, kind, off, t, v),
+                                      oracle.pairwise(op, p, kind, off, t, v, use_ref=True))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_integrals_match_exact_rationals(seed):
+    """L1, squared L2 and dot against exact Fraction sums (one rounding per term)."""
+    rng = np.random.default_rng(100 + seed)
+    pcfs = []
+    for m in rng.integers(1, 12, size=4):
+        t = np.concatenate([[0.0], np.cumsum(rng.exponential(1.0, size=m - 1))])
+        v = rng.standard_normal(m)
+        v[-1] = 0.0
+        pcfs.append(np.stack([t, v], 1))
+    for f in pcfs:
+        for g in pcfs:
+            S = len(f) + len(g)
+            for name, op, p in (("l1", oracle.OP_L1, 1.0), ("l2sq", oracle.OP_LP, 2.0),
+                                ("dot", oracle.OP_DOT, 1.0)):
+                exact = float(oracle.exact_integral(name, f, g))
+                got = oracle.integrate(op, p, f, g)
+                # error bound: a few roundings per term, relative to sum |term|
+                if name == "dot":
+                    af, ag = f.copy(), g.copy()
+                    af[:, 1], ag[:, 1] = np.abs(af[:, 1]), np.abs(ag[:, 1])
+                    scale = float(oracle.exact_integral("dot", af, ag))
+                else:
+                    scale = exact
+                assert abs(got - exact) <= 4 * S * 2.0 ** -52 * scale + 1e-300
+
+
+def test_common_grid_reduces_to_weighted_minkowski():
+    """PCFs on one shared grid: D = scipy pdist(V, minkowski, p, w=dt), K = V diag(dt) V^T."""
+    from scipy.spatial.distance import pdist as spdist
+
+    rng = np.random.default_rng(11)
+    n, m = 25, 40
+    grid = np.concatenate([[0.0], np.cumsum(rng.exponential(1.0, size=m - 1))])
+    V = rng.standard_normal((n, m))
+    V[:, -1] = 0.0
+    off = np.arange(0, n * m + 1, m, dtype=np.int64)
+    t = np.tile(grid, n)
+    v = V.reshape(-1)
+    dt = np.diff(grid)
+    Vc = V[:, :-1]
+    for p in (1.0, 2.0, 3.0):
+        op = oracle.OP_L1 if p == 1 else oracle.OP_LP
+        got = oracle.pairwise(op, p, oracle.KIND_PDIST, off, t, v)
+        exp = spdist(Vc, "minkowski", p=p, w=dt)
+        # scipy orders pairs (0,1),(0,2)...; the compressed layout is row-major lower
+        r, c = np.tril_indices(n, -1)
+        pos = {(i, j): k for k, (i, j) in enumerate(zip(*np.triu_indices(n, 1)))}
+        exp_c = np.array([exp[pos[(cc, rr)]] for rr, cc in zip(r, c)])
+        np.testing.assert_allclose(got, exp_c, rtol=1e-12)
+    K = oracle.pairwise(oracle.OP_DOT, 1.0, oracle.KIND_KERNEL, off, t, v)
+    np.testing.assert_allclose(_dense_from_kernel(K, n), (Vc * dt) @ Vc.T, rtol=1e-11, atol=1e-11)
+
+
+def test_invariants_polarization_scaling_redundant_breakpoint():
+    off, t, v = workloads.c2(30, seed=5)
+    n = len(off) - 1
+    D2 = _dense_from_pdist(oracle.pairwise(oracle.OP_LP, 2.0, 0, off, t, v), n)
+    K = _dense_from_kernel(oracle.pairwise(oracle.OP_DOT, 1.0, 1, off, t, v), n)
+    pol = np.diag(K)[:, None] + np.diag(K)[None, :] - 2 * K
+    np.testing.assert_allclose(D2 ** 2, pol, rtol=1e-8, atol=1e-8 * np.abs(K).max())
+    # value scaling: D(a f, a g) = |a| D(f, g)
+    D1 = oracle.pairwise(oracle.OP_L1, 1.0, 0, off, t, v)
+    np.testing.assert_allclose(oracle.pairwise(oracle.OP_L1, 1.0, 0, off, t, -2.5 * v), 2.5 * D1,
+                               rtol=1e-13)
+    # time scaling by a power of two is exact: D -> s D for L1
+    np.testing.assert_array_equal(oracle.pairwise(oracle.OP_L1, 1.0, 0, off, 4.0 * t, v), 4.0 * D1)
+    # inserting a redundant breakpoint (same value) changes nothing beyond rounding
+    f = np.stack([t[off[0]:off[1]], v[off[0]:off[1]]], 1)
+    g = np.stack([t[off[1]:off[2]], v[off[1]:off[2]]], 1)
+    mid = 0.5 * (f[1, 0] + f[2, 0])
+    f2 = np.insert(f, 2, [mid, f[1, 1]], axis=0)
+    assert oracle.integrate(oracle.OP_L1, 1.0, f2, g) == pytest.approx(
+        oracle.integrate(oracle.OP_L1, 1.0, f, g), rel=1e-14)
+    # cdist(X, X) equals pdist entries and is symmetric with zero diagonal
+    C = oracle.pairwise(oracle.OP_L1, 1.0, 2, off, t, v, off, t, v).reshape(n, n)
+    np.testing.assert_array_equal(C, C.T)
+    np.testing.assert_array_equal(np.diag(C), 0.0)
+    np.testing.assert_array_equal(C, _dense_from_pdist(D1, n))
+
+
+# ---------------------------------------------------------- gradients ------
+
+def _loss(op, kind, off, t, v, G):
+    p = 2.0 if op == oracle.OP_LP else 1.0
+    X = oracle.pairwise(op, p, kind, off, t, v, nthreads=1)
+    return float(np.dot(G, X))
+
+
+@pytest.mark.parametrize("op,kind", [(oracle.OP_LP, oracle.KIND_PDIST),
+                                     (oracle.OP_DOT, oracle.KIND_KERNEL)])
+def test_gradient_matches_central_differences(op, kind):
+    off, t, v, _ = workloads.c4(10)
+    n = len(off) - 1
+    rng = np.random.default_rng(3)
+    G = rng.uniform(0.0, 1.0, size=oracle.out_count(kind, n))
+    gv, gt = oracle.grad(op, kind, off, t, v, G)
+    h = 1e-6
+    last = set((off[1:] - 1).tolist())
+    first = set(off[:-1].tolist())
+    idx_v = [k for k in rng.choice(len(v), 40, replace=False) if k not in last]
+    for k in idx_v:
+        vp, vm = v.copy(), v.copy()
+        vp[k] += h
+        vm[k] -= h
+        fd = (_loss(op, kind, off, t, vp, G) - _loss(op, kind, off, t, vm, G)) / (2 * h)
+        assert gv[k] == pytest.approx(fd, rel=1e-5, abs=1e-6)
+    # interior breakpoints, away from any other breakpoint by more than 10 h
+    allt = np.sort(t)
+    idx_t = []
+    for k in rng.permutation(len(t)):
+        if k in first:
+            continue
+        i = np.searchsorted(off, k, side="right") - 1
+        own = t[off[i]:off[i + 1]]
+        others = np.concatenate([t[:off[i]], t[off[i + 1]:]])
+        gap = np.min(np.abs(others - t[k])) if len(others) else np.inf
+        j = k - off[i]
+        near_own = min(own[j] - own[j - 1], own[j + 1] - own[j] if j + 1 < len(own) else np.inf)
+        if gap > 1e-4 and near_own > 1e-4:
+            idx_t.append(k)
+        if len(idx_t) == 30:
+            break
+    for k in idx_t:
+        tp, tm = t.copy(), t.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (_loss(op, kind, off, tp, v, G) - _loss(op, kind, off, tm, v, G)) / (2 * h)
+        assert gt[k] == pytest.approx(fd, rel=1e-5, abs=1e-6)
+    del allt
+
+
+def test_gradient_closed_forms_constant_and_self_pair():
+    # K_00 = int f^2 for f = a on [0, T): dK/da = 2 a T, dK/dT = a^2 (breakpoint at T)
+    off, t, v = _csr([[[0.0, 3.0], [2.0, 0.0]]])
+    gv, gt = oracle.grad(oracle.OP_DOT, oracle.KIND_KERNEL, off, t, v, np.array([1.0]))
+    assert gv[0] == pytest.approx(2 * 3.0 * 2.0)
+    assert gt[1] == pytest.approx(3.0 ** 2)
+    assert gt[0] == 0.0
+    # D = |a - b| sqrt(T) for constants a, b on [0, T): dD/da = sign(a - b) sqrt(T)
+    off, t, v = _csr([[[0.0, 4.0], [9.0, 0.0]], [[0.0, 1.0], [9.0, 0.0]]])
+    gv, gt = oracle.grad(oracle.OP_LP, oracle.KIND_PDIST, off, t, v, np.array([1.0]))
+    assert gv[0] == pytest.approx(3.0)
+    assert gv[2] == pytest.approx(-3.0)
+    # Both PCFs end at T = 9 (a tie). Moving the shared end time as one variable
+    # gives dS/dT = (a - b)^2 = 9 and dD/dT = 9 / (2 D) = 0.5; the two one-sided
+    # halves of the tie convention add up to it.
+    assert gt[1] + gt[3] == pytest.approx(0.5)
