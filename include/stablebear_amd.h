@@ -110,6 +110,8 @@ int sbamd_device_count(int* count);
 int sbamd_ctx_create(int device, sbamd_ctx** out);
 int sbamd_ctx_destroy(sbamd_ctx* ctx);
 int sbamd_ctx_synchronize(sbamd_ctx* ctx);
+/* The context's own (non-blocking) stream, for callers without a stream. */
+int sbamd_ctx_stream(sbamd_ctx* ctx, void** stream);
 
 /* ---- device-resident PCF sets ----
  * Validates, drops zero-length segments, sorts PCFs by record count and
@@ -143,8 +145,8 @@ int sbamd_plan_out_offset(const sbamd_plan* plan, int32_t problem, uint64_t* off
  * partials into FP64 per phase (default 8192). */
 int sbamd_plan_set_fold_threshold(sbamd_plan* plan, int64_t segments);
 
-/* Enqueue the whole forward computation on `stream` (hipStream_t; NULL = the
- * context stream). out_device: device buffer of plan out_count elements of
+/* Enqueue the whole forward computation on `stream` (a hipStream_t; NULL is
+ * the null stream, as in HIP). out_device: device buffer of plan out_count elements of
  * out_dtype. row_sum_device (optional, DENSE only): n_rows FP64 sums of each
  * output row (for row means). Asynchronous; capturable into a hipGraph.
  * Replaces launch_pcf_block_integrate + the block pipeline's scatter

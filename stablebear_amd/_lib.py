@@ -67,6 +67,7 @@ _SIGS = {
     "sbamd_ctx_create": (None, [ctypes.c_int, ctypes.POINTER(vp)]),
     "sbamd_ctx_destroy": (None, [vp]),
     "sbamd_ctx_synchronize": (None, [vp]),
+    "sbamd_ctx_stream": (None, [vp, ctypes.POINTER(vp)]),
     "sbamd_set_create": (None, [vp, ctypes.POINTER(PcfSet), ctypes.c_int32, ctypes.POINTER(vp)]),
     "sbamd_set_destroy": (None, [vp]),
     "sbamd_set_info": (None, [vp, c_i64p, c_i64p, c_i64p]),

@@ -98,10 +98,10 @@ namespace sbamd
 
     void use_device(const Ctx* ctx) { SBAMD_HIP(hipSetDevice(ctx->device)); }
 
-    hipStream_t pick_stream(const Ctx* ctx, void* stream)
-    {
-      return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-    }
+    // Stream arguments are hipStream_t values as given: NULL is the null
+    // (legacy default) stream, as everywhere in HIP. The context's own stream
+    // is available through sbamd_ctx_stream().
+    hipStream_t pick_stream(const Ctx*, void* stream) { return static_cast<hipStream_t>(stream); }
 
     template <typename T>
     void upload(DeviceBuffer& buf, const std::vector<T>& v)
@@ -419,6 +419,15 @@ extern "C"
       need(ctx, "ctx");
       use_device(ctx);
       SBAMD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+  }
+
+  int sbamd_ctx_stream(sbamd_ctx* ctx, void** stream)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      need(stream, "stream");
+      *stream = static_cast<void*>(ctx->stream);
     });
   }
 

@@ -115,10 +115,12 @@ namespace sbamd
     template <typename TT>
     __device__ __forceinline__ double finish(TT tail, float w)
     {
-      // The final rectangle [t, MAX) is formed in FP64: its length overflows
-      // FP32, and 0 * inf would turn a zero tail into NaN.
-      double r = big + (static_cast<double>(s) - static_cast<double>(c));
-      return r + static_cast<double>(tail) * static_cast<double>(w);
+      // The final rectangle [t, MAX) is formed in the time type, whose MAX
+      // bounds the domain (FLT_MAX or DBL_MAX), so a non-zero tail overflows
+      // exactly as the reference's working type does; a zero tail adds 0.
+      const double r = big + (static_cast<double>(s) - static_cast<double>(c));
+      const TT term = tail * static_cast<TT>(w);
+      return r + static_cast<double>(term);
     }
   };
 

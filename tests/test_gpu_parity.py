@@ -150,11 +150,14 @@ def test_pcf32_matches_reference_float_path(gpu, name, oname):
 def _assert_fp32c(got, exp, signed=False, scale=None, tol=1e-6):
     got = np.asarray(got, dtype=np.float64)
     exp = np.asarray(exp, dtype=np.float64)
-    if signed:
-        s = scale if scale is not None else np.maximum(np.abs(exp), 1e-30)
-        err = np.abs(got - exp) / s
-    else:
-        err = np.abs(got - exp) / np.where(exp == 0, 1.0, np.abs(exp))
+    same = got == exp  # covers equal infinities (PCFs ending at different values)
+    with np.errstate(invalid="ignore"):
+        if signed:
+            s = scale if scale is not None else np.maximum(np.abs(exp), 1e-30)
+            err = np.where(same, 0.0, np.abs(got - exp) / s)
+        else:
+            err = np.where(same, 0.0, np.abs(got - exp) / np.where(exp == 0, 1.0, np.abs(exp)))
+    assert not np.any(np.isnan(err)), "non-finite mismatch"
     assert err.max() <= tol, f"max rel err {err.max():.3e}"
     return float(err.max())
 
