@@ -871,13 +871,30 @@ extern "C"
         int R, C, cap;
       };
       std::vector<Cand> cands;
-      const int sides[] = {8, 16};
-      const int caps[] = {32, 64, 128, 256};
-      const size_t budget = 64 * 1024;
+      // 16 x 16 tiles (256 lanes) with LDS windows small enough for several
+      // resident workgroups; 8 x 8 only where a plan is too small to fill
+      // the device with 16 x 16 tiles.
+      const int sides[] = {16, 8};
+      const int caps[] = {24, 32, 48, 64, 96};
+      const size_t budget = 48 * 1024;
+      uint64_t big_tiles = 0; // 16 x 16 tiles of the plan
+      for (size_t b = 0; b < plan->sets.size(); ++b)
+      {
+        const uint64_t nr = (static_cast<uint64_t>(plan->sets[b]->n) + 15) / 16;
+        if (plan->cdist)
+        {
+          big_tiles = nr * ((static_cast<uint64_t>(plan->sets[1]->n) + 15) / 16);
+          break;
+        }
+        big_tiles += nr * (nr + 1) / 2;
+      }
+      const bool small = big_tiles < 4ull * static_cast<uint64_t>(plan->ctx->cu_count);
       for (int R : sides)
         for (int C : sides)
         {
           if (!plan->cdist && R != C)
+            continue;
+          if ((R < 16 || C < 16) && !small)
             continue;
           for (int cap : caps)
           {

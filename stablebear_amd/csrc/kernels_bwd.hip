@@ -15,9 +15,9 @@
 //
 // Value gradients of the current record accumulate in registers and are
 // flushed when the record is left; flushes and time gradients go to FP64 LDS
-// slots of the staged records, which are flushed to global memory (one atomic
-// per staged record and phase). Records staged in an earlier phase are
-// flushed straight to global memory.
+// slots beside the staged records, flushed to global memory once per staged
+// record and phase. A record staged in an earlier phase is flushed straight
+// to global memory.
 #include "walk_common.hpp"
 
 namespace sbamd
@@ -30,65 +30,34 @@ namespace sbamd
       atomicAdd(static_cast<float*>(base) + idx, static_cast<float>(x));
   }
 
-  template <typename TT, typename TV, int OP>
+  template <typename TV, int OP>
   __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdParams B)
   {
     extern __shared__ __align__(16) unsigned char smem[];
+    using TT = double;
     using L = Lim<TT>;
-    using Bits = typename L::Bits;
+    using RecT = Rec<TT, TV>;
     const FwdParams& P = B.f;
 
-    const int NS = P.R + P.C;
-    const int stride = P.stride;
-    const int cap = P.cap;
-    const LdsLayout lay = lds_layout(sizeof(TT), sizeof(TV), NS, stride, cap, 16);
-    TT* s_tend = reinterpret_cast<TT*>(smem + lay.tend_off);
-    TV* s_val = reinterpret_cast<TV*>(smem + lay.val_off);
-    int64_t* s_base = reinterpret_cast<int64_t*>(smem + lay.base_off);
-    int64_t* s_m = reinterpret_cast<int64_t*>(smem + lay.m_off);
-    int64_t* s_cur = reinterpret_cast<int64_t*>(smem + lay.cur_off);
-    int64_t* s_next = reinterpret_cast<int64_t*>(smem + lay.next_off);
-    TT* s_tstart = reinterpret_cast<TT*>(smem + lay.tstart_off);
-    Bits* s_tb = reinterpret_cast<Bits*>(smem + lay.tb_off);
-    double* s_gv = reinterpret_cast<double*>(smem + lay.extra_off);
-    double* s_gt = s_gv + cap * stride;
-
-    const TT* __restrict__ g_t = static_cast<const TT*>(P.r_tend);
-    const TV* __restrict__ g_v = static_cast<const TV*>(P.r_val);
+    // 16 bytes of gradient slots (value, time) per staged record
+    const LdsLayout lay = lds_layout(sizeof(TT), P.R, P.C, P.cap, 16);
+    const TileLds<TT, TV> S(smem, lay);
+    static_assert(sizeof(RecT) == 16, "gradient slots mirror 16-byte records");
+    constexpr uint32_t SF = sizeof(RecT); // record stride of every slot
+    constexpr uint32_t SG = SF;
+    const uint32_t GOFF = static_cast<uint32_t>(lay.extra_off);
     const int64_t* __restrict__ src = B.r_src;
 
     const int tid = threadIdx.x;
+    const int nthreads = blockDim.x;
+    const int NS = P.R + P.C;
 
     for (int64_t ti = blockIdx.x; ti < P.ntiles; ti += gridDim.x)
     {
       const Tile tl = P.tiles[ti];
       const Problem pr = P.probs[tl.prob];
 
-      if (tid < NS)
-      {
-        const bool isrow = tid < P.R;
-        const int k = isrow ? tid : tid - P.R;
-        const int lim = isrow ? tl.nr : tl.nc;
-        if (k < lim)
-        {
-          const int64_t g = isrow ? pr.row_base + tl.r0 + k : pr.col_base + tl.c0 + k;
-          const int64_t b = P.r_off[g];
-          s_base[tid] = b;
-          s_m[tid] = P.r_off[g + 1] - b;
-        }
-        else
-        {
-          s_base[tid] = 0;
-          s_m[tid] = 0;
-        }
-        s_next[tid] = 0;
-        s_tstart[tid] = TT(0);
-      }
-      if (tid == 0)
-      {
-        s_tb[0] = L::bits(L::inf());
-        s_tb[1] = L::bits(L::inf());
-      }
+      tile_begin(S, P, tl, pr, tid);
       __syncthreads();
 
       const int lr = tid / P.C;
@@ -96,8 +65,6 @@ namespace sbamd
       bool valid = tid < P.R * P.C && lr < tl.nr && lc < tl.nc;
       if (valid && (tl.flags & TILE_DIAG))
         valid = P.include_diag ? (lr >= lc) : (lr > lc);
-      const int slot_f = lr;
-      const int slot_g = P.R + lc;
 
       // Pair weight.
       double w = 0.0;
@@ -121,6 +88,9 @@ namespace sbamd
         if (w == 0.0)
           valid = false;
       }
+      const int slot_f = valid ? lr : 0;
+      const int slot_g = valid ? P.R + lc : P.R;
+      const double scale = OP == OPK_DOT ? w : 2.0 * w;
 
       TT t = TT(0);
       TT tf = L::inf(), tg = L::inf();
@@ -130,77 +100,39 @@ namespace sbamd
 
       for (int phase = 0;; ++phase)
       {
-        if (tid < NS)
-        {
-          const int64_t cur = s_next[tid];
-          s_cur[tid] = cur;
-          const int64_t rem = s_m[tid] - cur;
-          TT tbs = L::inf();
-          if (rem > cap)
-            tbs = g_t[s_base[tid] + cur + cap - 1];
-          atomicMin(&s_tb[phase & 1], L::bits(tbs));
-        }
-        for (int idx = tid; idx < NS * cap; idx += blockDim.x)
-        {
-          const int s = idx / cap;
-          const int j = idx - s * cap;
-          const int64_t k = s_next[s] + j;
-          if (k < s_m[s])
-          {
-            const int64_t gk = s_base[s] + k;
-            s_tend[j * stride + s] = g_t[gk];
-            s_val[j * stride + s] = g_v[gk];
-          }
-          s_gv[j * stride + s] = 0.0;
-          s_gt[j * stride + s] = 0.0;
-        }
-        if (tid == 0)
-          s_tb[(phase + 1) & 1] = L::bits(L::inf());
+        phase_stage(S, P, phase, tid, nthreads);
+        for (int k = tid; k < lay.rec_end / 16; k += nthreads)
+          *reinterpret_cast<double2*>(smem + GOFF + 16 * k) = make_double2(0.0, 0.0);
         __syncthreads();
-
-        const TT Tb = L::from_bits(s_tb[phase & 1]);
-
-        if (tid < NS)
-        {
-          const int64_t cur = s_cur[tid];
-          const int64_t rem = s_m[tid] - cur;
-          const int lim = rem < cap ? static_cast<int>(rem) : cap;
-          int lo = 0, hi = lim;
-          while (lo < hi)
-          {
-            const int mid = (lo + hi) >> 1;
-            const TT ts = mid == 0 ? s_tstart[tid] : s_tend[(mid - 1) * stride + tid];
-            if (ts < Tb)
-              lo = mid + 1;
-            else
-              hi = mid;
-          }
-          s_next[tid] = cur + lo;
-          if (lo > 0)
-            s_tstart[tid] = s_tend[(lo - 1) * stride + tid];
-        }
+        const TT Tb = L::from_bits(S.tb[phase & 1]);
+        phase_advance_cursor(S, P, Tb, tid);
 
         if (valid)
         {
-          const int64_t cf = s_cur[slot_f];
-          const int64_t cg = s_cur[slot_g];
+          const int64_t cf = S.cur[slot_f];
+          const int64_t cg = S.cur[slot_g];
+          const uint32_t bf = S.lbase[slot_f];
+          const uint32_t bg = S.lbase[slot_g];
+          const int64_t basef = S.base[slot_f];
+          const int64_t baseg = S.base[slot_g];
           if (phase == 0)
           {
-            tf = s_tend[slot_f];
-            fv = s_val[slot_f];
-            tg = s_tend[slot_g];
-            gv = s_val[slot_g];
+            const RecT rf = S.rec(bf);
+            const RecT rg = S.rec(bg);
+            tf = rf.t;
+            fv = rf.v;
+            tg = rg.t;
+            gv = rg.v;
           }
-          int lf = static_cast<int>(nf - cf);
-          int lg = static_cast<int>(ng - cg);
-          const double scale = OP == OPK_DOT ? w : 2.0 * w;
+          uint32_t af_addr = bf + static_cast<uint32_t>(nf - cf) * SF;
+          uint32_t ag_addr = bg + static_cast<uint32_t>(ng - cg) * SG;
           for (;;)
           {
             const bool af = tf <= tg;
             const TT tn = af ? tf : tg;
             if (!(tn < Tb))
               break;
-            const double len = static_cast<double>(tn - t);
+            const double len = tn - t;
             if constexpr (OP == OPK_DOT)
             {
               accf += static_cast<double>(gv) * len;
@@ -213,60 +145,55 @@ namespace sbamd
               accg -= d * len;
             }
             t = tn;
-            const int s = af ? slot_f : slot_g;
-            const int lj = af ? lf : lg;
-            // flush the value gradient of the record being left (local lj - 1)
+            const uint32_t a = af ? af_addr : ag_addr;
+            const uint32_t st = af ? SF : SG;
+            const uint32_t b0 = af ? bf : bg;
+            const RecT r = S.rec(a);
+            // flush the value gradient of the record being left
             const double gflush = (af ? accf : accg) * scale;
-            if (lj - 1 >= 0)
-              atomicAdd(&s_gv[(lj - 1) * stride + s], gflush);
+            if (a - b0 >= st)
+              atomicAdd(reinterpret_cast<double*>(smem + GOFF + a - st), gflush);
             else
-              gadd(B.grad_v, src[s_base[s] + (af ? cf : cg) + lj - 1], gflush, B.grad_f64);
-            const TT tnew = s_tend[lj * stride + s];
-            const TV vnew = s_val[lj * stride + s];
+              gadd(B.grad_v, src[(af ? basef + cf : baseg + cg) - 1], gflush, B.grad_f64);
+            // time gradient of the breakpoint just crossed (start of record r)
             const double vold = af ? static_cast<double>(fv) : static_cast<double>(gv);
             const double c = af ? static_cast<double>(gv) : static_cast<double>(fv);
-            const double vn = static_cast<double>(vnew);
+            const double vn = static_cast<double>(r.v);
             double dt_grad;
             if constexpr (OP == OPK_DOT)
               dt_grad = (vold - vn) * c;
             else
               dt_grad = (vold - c) * (vold - c) - (vn - c) * (vn - c);
-            atomicAdd(&s_gt[lj * stride + s], dt_grad * w);
-            if (af)
-            {
-              tf = tnew;
-              fv = vnew;
-              ++lf;
-              accf = 0.0;
-            }
-            else
-            {
-              tg = tnew;
-              gv = vnew;
-              ++lg;
-              accg = 0.0;
-            }
+            atomicAdd(reinterpret_cast<double*>(smem + GOFF + a + 8), dt_grad * w);
+            af_addr = af ? af_addr + SF : af_addr;
+            ag_addr = af ? ag_addr : ag_addr + SG;
+            tf = af ? r.t : tf;
+            fv = af ? r.v : fv;
+            tg = af ? tg : r.t;
+            gv = af ? gv : r.v;
+            accf = af ? 0.0 : accf;
+            accg = af ? accg : 0.0;
           }
-          nf = cf + lf;
-          ng = cg + lg;
+          nf = cf + (af_addr - bf) / SF;
+          ng = cg + (ag_addr - bg) / SG;
         }
         __syncthreads();
 
         // flush this phase's record slots
-        for (int idx = tid; idx < NS * cap; idx += blockDim.x)
+        for (int idx = tid; idx < NS * P.cap; idx += nthreads)
         {
-          const int s = idx / cap;
-          const int j = idx - s * cap;
-          const int64_t k = s_cur[s] + j;
-          if (k < s_next[s])
+          const int s = idx / P.cap;
+          const int j = idx - s * P.cap;
+          const int64_t k = S.cur[s] + j;
+          if (k < S.next[s])
           {
-            const int64_t pt = src[s_base[s] + k];
-            const double a = s_gv[j * stride + s];
-            const double b2 = s_gt[j * stride + s];
-            if (a != 0.0)
-              gadd(B.grad_v, pt, a, B.grad_f64);
-            if (b2 != 0.0 && k > 0)
-              gadd(B.grad_t, pt, b2, B.grad_f64);
+            const uint32_t a = S.lbase[s] + static_cast<uint32_t>(j) * S.lstride[s];
+            const double2 g2 = *reinterpret_cast<const double2*>(smem + GOFF + a);
+            const int64_t pt = src[S.base[s] + k];
+            if (g2.x != 0.0)
+              gadd(B.grad_v, pt, g2.x, B.grad_f64);
+            if (g2.y != 0.0 && k > 0)
+              gadd(B.grad_t, pt, g2.y, B.grad_f64);
           }
         }
         __syncthreads();
@@ -277,7 +204,7 @@ namespace sbamd
       if (valid)
       {
         // final rectangle [t, MAX) and flush of the last records
-        const double len = static_cast<double>(L::max() - t);
+        const double len = L::max() - t;
         if constexpr (OP == OPK_DOT)
         {
           if (gv != TV(0))
@@ -294,9 +221,8 @@ namespace sbamd
             accg -= d * len;
           }
         }
-        const double scale = OP == OPK_DOT ? w : 2.0 * w;
-        gadd(B.grad_v, src[s_base[slot_f] + nf - 1], accf * scale, B.grad_f64);
-        gadd(B.grad_v, src[s_base[slot_g] + ng - 1], accg * scale, B.grad_f64);
+        gadd(B.grad_v, src[S.base[slot_f] + nf - 1], accf * scale, B.grad_f64);
+        gadd(B.grad_v, src[S.base[slot_g] + ng - 1], accg * scale, B.grad_f64);
       }
       __syncthreads();
     }
@@ -304,33 +230,31 @@ namespace sbamd
 
   size_t bwd_lds_bytes(int tv_f64, int R, int C, int cap)
   {
-    const int NS = R + C;
-    return static_cast<size_t>(lds_layout(8, tv_f64 ? 8 : 4, NS, NS, cap, 16).total);
+    (void)tv_f64;
+    return static_cast<size_t>(lds_layout(8, R, C, cap, 16).total);
   }
 
   void launch_backward(const BwdParams& prm, int op, hipStream_t stream)
   {
     const FwdParams& P = prm.f;
-    const int NS = P.R + P.C;
     const unsigned grid = static_cast<unsigned>(P.ntiles < 0x7fffffff ? P.ntiles : 0x7fffffff);
     if (grid == 0)
       return;
-    const int tt = 8;
+    const size_t lds = bwd_lds_bytes(prm.val_f64, P.R, P.C, P.cap);
+    const dim3 block(tile_block_threads(P.R, P.C));
     if (prm.val_f64)
     {
-      const size_t lds = static_cast<size_t>(lds_layout(tt, 8, NS, P.stride, P.cap, 16).total);
       if (op == OPK_DOT)
-        hipLaunchKernelGGL((bwd_tile_kernel<double, double, OPK_DOT>), dim3(grid), dim3(tile_block_threads(P.R, P.C)), lds, stream, prm);
+        hipLaunchKernelGGL((bwd_tile_kernel<double, OPK_DOT>), dim3(grid), block, lds, stream, prm);
       else
-        hipLaunchKernelGGL((bwd_tile_kernel<double, double, OPK_L2>), dim3(grid), dim3(tile_block_threads(P.R, P.C)), lds, stream, prm);
+        hipLaunchKernelGGL((bwd_tile_kernel<double, OPK_L2>), dim3(grid), block, lds, stream, prm);
     }
     else
     {
-      const size_t lds = static_cast<size_t>(lds_layout(tt, 4, NS, P.stride, P.cap, 16).total);
       if (op == OPK_DOT)
-        hipLaunchKernelGGL((bwd_tile_kernel<double, float, OPK_DOT>), dim3(grid), dim3(tile_block_threads(P.R, P.C)), lds, stream, prm);
+        hipLaunchKernelGGL((bwd_tile_kernel<float, OPK_DOT>), dim3(grid), block, lds, stream, prm);
       else
-        hipLaunchKernelGGL((bwd_tile_kernel<double, float, OPK_L2>), dim3(grid), dim3(tile_block_threads(P.R, P.C)), lds, stream, prm);
+        hipLaunchKernelGGL((bwd_tile_kernel<float, OPK_L2>), dim3(grid), block, lds, stream, prm);
     }
     SBAMD_HIP(hipGetLastError());
   }

@@ -7,38 +7,30 @@
 // in LDS in time phases: every staged record serves all R (or C) pairs that
 // use it. A phase ends at the time Tb where the first tile PCF would overflow
 // `cap` staged records; each lane walks its pair up to Tb, then the next
-// records are staged. Records are stored slot-interleaved ([record][slot]) so
-// lanes reading different PCFs hit different LDS banks.
+// records are staged.
+//
+// The walk step is branch-free: one compare picks the side whose record ends
+// first, one LDS load fetches that side's next record, and selects update the
+// state, so a wavefront issues the same instruction stream for every lane.
 #include "walk_common.hpp"
 
 namespace sbamd
 {
   template <typename TT, typename TV, int OP, bool KAHAN>
-  __global__ __launch_bounds__(256) void fwd_tile_kernel(FwdParams P)
+  // 8 waves per SIMD (at most 64 VGPRs): the walk is latency-bound on its
+  // dependent LDS loads, so residency matters more than registers.
+  __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void fwd_tile_kernel(FwdParams P)
   {
     extern __shared__ __align__(16) unsigned char smem[];
     using L = Lim<TT>;
-    using Bits = typename L::Bits;
+    using RecT = Rec<TT, TV>;
 
-    const int NS = P.R + P.C;
-    const int stride = P.stride;
-    const int cap = P.cap;
-    const LdsLayout lay = lds_layout(sizeof(TT), sizeof(TV), NS, stride, cap, 0);
-    TT* s_tend = reinterpret_cast<TT*>(smem + lay.tend_off);
-    TV* s_val = reinterpret_cast<TV*>(smem + lay.val_off);
-    int64_t* s_base = reinterpret_cast<int64_t*>(smem + lay.base_off);
-    int64_t* s_m = reinterpret_cast<int64_t*>(smem + lay.m_off);
-    int64_t* s_cur = reinterpret_cast<int64_t*>(smem + lay.cur_off);
-    int64_t* s_next = reinterpret_cast<int64_t*>(smem + lay.next_off);
-    TT* s_tstart = reinterpret_cast<TT*>(smem + lay.tstart_off);
-    Bits* s_tb = reinterpret_cast<Bits*>(smem + lay.tb_off);
-
-    const TT* __restrict__ g_rt = static_cast<const TT*>(P.r_tend);
-    const TV* __restrict__ g_rv = static_cast<const TV*>(P.r_val);
-    const TT* __restrict__ g_ct = static_cast<const TT*>(P.c_tend);
-    const TV* __restrict__ g_cv = static_cast<const TV*>(P.c_val);
+    const LdsLayout lay = lds_layout(sizeof(TT), P.R, P.C, P.cap, 0);
+    const TileLds<TT, TV> S(smem, lay);
+    constexpr uint32_t SF = sizeof(RecT); // record stride within a slot
 
     const int tid = threadIdx.x;
+    const int nthreads = blockDim.x;
     const TV p = static_cast<TV>(P.p);
 
     for (int64_t ti = blockIdx.x; ti < P.ntiles; ti += gridDim.x)
@@ -47,32 +39,7 @@ namespace sbamd
       const Problem pr = P.probs[tl.prob];
       const bool fold = (tl.flags & TILE_FOLD) != 0;
 
-      if (tid < NS)
-      {
-        const bool isrow = tid < P.R;
-        const int k = isrow ? tid : tid - P.R;
-        const int lim = isrow ? tl.nr : tl.nc;
-        const int64_t* off = isrow ? P.r_off : P.c_off;
-        if (k < lim)
-        {
-          const int64_t g = isrow ? pr.row_base + tl.r0 + k : pr.col_base + tl.c0 + k;
-          const int64_t b = off[g];
-          s_base[tid] = b;
-          s_m[tid] = off[g + 1] - b;
-        }
-        else
-        {
-          s_base[tid] = 0;
-          s_m[tid] = 0;
-        }
-        s_next[tid] = 0;
-        s_tstart[tid] = TT(0);
-      }
-      if (tid == 0)
-      {
-        s_tb[0] = L::bits(L::inf());
-        s_tb[1] = L::bits(L::inf());
-      }
+      tile_begin(S, P, tl, pr, tid);
       __syncthreads();
 
       // Pair of this lane (sorted positions a = r0 + lr, b = c0 + lc).
@@ -81,114 +48,81 @@ namespace sbamd
       bool valid = tid < P.R * P.C && lr < tl.nr && lc < tl.nc;
       if (valid && (tl.flags & TILE_DIAG))
         valid = P.include_diag ? (lr >= lc) : (lr > lc);
-      const int slot_f = lr;
-      const int slot_g = P.R + lc;
+      const int slot_f = valid ? lr : 0;
+      const int slot_g = valid ? P.R + lc : P.R;
 
+      // The two current records of the pair, ordered by end time: the next
+      // union breakpoint is always lo.t, and the side that owns lo advances.
+      // (Which PCF lo belongs to does not matter to a symmetric pair op.)
+      // lo_p / hi_p: LDS address of the owning PCF's next record; between
+      // phases they are kept as (slot, record index) instead.
       TT t = TT(0);
-      TT tf = L::inf(), tg = L::inf();
-      TV fv = TV(0), gv = TV(0);
-      int64_t nf = 1, ng = 1; // global record index (within the PCF) of the next record
+      TT lo_t = L::inf(), hi_t = L::inf();
+      TV lo_v = TV(0), hi_v = TV(0);
+      int lo_s = slot_f, hi_s = slot_g;  // owning slots
+      int64_t lo_k = 1, hi_k = 1;        // record index of each owner's next record
       using Acc = typename std::conditional<KAHAN, KahanAcc, PlainAcc<TV>>::type;
       Acc acc;
 
       for (int phase = 0;; ++phase)
       {
-        // ---- stage the records that start before this phase's end time ----
-        if (tid < NS)
-        {
-          const int64_t cur = s_next[tid];
-          s_cur[tid] = cur;
-          const int64_t rem = s_m[tid] - cur;
-          TT tbs = L::inf();
-          if (rem > cap)
-          {
-            const TT* gt = tid < P.R ? g_rt : g_ct;
-            tbs = gt[s_base[tid] + cur + cap - 1];
-          }
-          atomicMin(&s_tb[phase & 1], L::bits(tbs));
-        }
-        for (int idx = tid; idx < NS * cap; idx += blockDim.x)
-        {
-          const int s = idx / cap;
-          const int j = idx - s * cap;
-          const int64_t k = s_next[s] + j;
-          if (k < s_m[s])
-          {
-            const bool isrow = s < P.R;
-            const int64_t gk = s_base[s] + k;
-            s_tend[j * stride + s] = isrow ? g_rt[gk] : g_ct[gk];
-            s_val[j * stride + s] = isrow ? g_rv[gk] : g_cv[gk];
-          }
-        }
-        if (tid == 0)
-          s_tb[(phase + 1) & 1] = L::bits(L::inf());
+        phase_stage(S, P, phase, tid, nthreads);
         __syncthreads();
+        const TT Tb = L::from_bits(S.tb[phase & 1]);
+        phase_advance_cursor(S, P, Tb, tid);
 
-        const TT Tb = L::from_bits(s_tb[phase & 1]);
-
-        // ---- next cursor of each slot: first staged record starting at or after Tb ----
-        if (tid < NS)
-        {
-          const int64_t cur = s_cur[tid];
-          const int64_t rem = s_m[tid] - cur;
-          const int lim = rem < cap ? static_cast<int>(rem) : cap;
-          int lo = 0, hi = lim;
-          while (lo < hi)
-          {
-            const int mid = (lo + hi) >> 1;
-            const TT ts = mid == 0 ? s_tstart[tid] : s_tend[(mid - 1) * stride + tid];
-            if (ts < Tb)
-              lo = mid + 1;
-            else
-              hi = mid;
-          }
-          s_next[tid] = cur + lo;
-          if (lo > 0)
-            s_tstart[tid] = s_tend[(lo - 1) * stride + tid];
-        }
-
-        // ---- walk this lane's pair up to Tb ----
         if (valid)
         {
-          const int64_t cf = s_cur[slot_f];
-          const int64_t cg = s_cur[slot_g];
           if (phase == 0)
           {
-            tf = s_tend[slot_f];
-            fv = s_val[slot_f];
-            tg = s_tend[slot_g];
-            gv = s_val[slot_g];
+            const RecT rf = S.rec(S.lbase[slot_f]);
+            const RecT rg = S.rec(S.lbase[slot_g]);
+            const bool k = rf.t <= rg.t;
+            lo_t = k ? rf.t : rg.t;
+            lo_v = k ? rf.v : rg.v;
+            hi_t = k ? rg.t : rf.t;
+            hi_v = k ? rg.v : rf.v;
+            lo_s = k ? slot_f : slot_g;
+            hi_s = k ? slot_g : slot_f;
           }
-          int lf = static_cast<int>(nf - cf);
-          int lg = static_cast<int>(ng - cg);
-          for (;;)
+          const int64_t lo_c = S.cur[lo_s], hi_c = S.cur[hi_s];
+          const RecT* const recs = reinterpret_cast<const RecT*>(smem);
+          const RecT* lo_p = recs + (S.lbase[lo_s] / SF + static_cast<uint32_t>(lo_k - lo_c));
+          const RecT* hi_p = recs + (S.lbase[hi_s] / SF + static_cast<uint32_t>(hi_k - hi_c));
+          // One merge step: accumulate [t, lo.t), then replace lo by the next
+          // record of its PCF and restore the order. Two steps per trip let
+          // the registers of t, lo and the accumulator alternate roles.
+          auto step = [&]() {
+            acc.add(static_cast<TV>(lo_t - t), pair_op<OP, TV>(lo_v, hi_v, p));
+            t = lo_t;
+            const RecT r = *lo_p;
+            const RecT* np = lo_p + 1;
+            const bool k = r.t <= hi_t;
+            const TT nlt = k ? r.t : hi_t;
+            const TV nlv = k ? r.v : hi_v;
+            const RecT* nlp = k ? np : hi_p;
+            hi_t = k ? hi_t : r.t;
+            hi_v = k ? hi_v : r.v;
+            hi_p = k ? hi_p : np;
+            lo_t = nlt;
+            lo_v = nlv;
+            lo_p = nlp;
+          };
+          while (lo_t < Tb)
           {
-            const bool af = tf <= tg;
-            const TT tn = af ? tf : tg;
-            if (!(tn < Tb))
+            step();
+            if (!(lo_t < Tb))
               break;
-            const TV dt = static_cast<TV>(tn - t);
-            acc.add(dt, pair_op<OP, TV>(fv, gv, p));
-            t = tn;
-            const int j = af ? lf : lg;
-            const int s = af ? slot_f : slot_g;
-            const TT tnew = s_tend[j * stride + s];
-            const TV vnew = s_val[j * stride + s];
-            if (af)
-            {
-              tf = tnew;
-              fv = vnew;
-              ++lf;
-            }
-            else
-            {
-              tg = tnew;
-              gv = vnew;
-              ++lg;
-            }
+            step();
           }
-          nf = cf + lf;
-          ng = cg + lg;
+          // back to (slot, record index): slots are slot_recs records apart
+          const uint32_t SR = static_cast<uint32_t>(slot_recs(P.cap));
+          const uint32_t lo_i = static_cast<uint32_t>(lo_p - recs);
+          const uint32_t hi_i = static_cast<uint32_t>(hi_p - recs);
+          lo_s = static_cast<int>(lo_i / SR);
+          hi_s = static_cast<int>(hi_i / SR);
+          lo_k = S.cur[lo_s] + (lo_i - S.lbase[lo_s] / SF);
+          hi_k = S.cur[hi_s] + (hi_i - S.lbase[hi_s] / SF);
           if (fold)
             acc.fold();
         }
@@ -199,7 +133,7 @@ namespace sbamd
 
       if (valid)
       {
-        const TV w = pair_op<OP, TV>(fv, gv, p);
+        const TV w = pair_op<OP, TV>(lo_v, hi_v, p);
         const TT tail = L::max() - t;
         const double raw = acc.finish(tail, w);
         const double res = finalize<OP>(raw, P.p);
@@ -226,16 +160,14 @@ namespace sbamd
 
   size_t fwd_lds_bytes(int tt_f64, int tv_f64, int R, int C, int cap)
   {
-    const int NS = R + C;
-    return static_cast<size_t>(lds_layout(tt_f64 ? 8 : 4, tv_f64 ? 8 : 4, NS, NS, cap, 0).total);
+    (void)tv_f64;
+    return static_cast<size_t>(lds_layout(tt_f64 ? 8 : 4, R, C, cap, 0).total);
   }
 
   template <typename TT, typename TV, bool KAHAN>
   static void launch_typed(const FwdParams& prm, int op, hipStream_t stream)
   {
-    const int NS = prm.R + prm.C;
-    const size_t lds = static_cast<size_t>(
-        lds_layout(sizeof(TT), sizeof(TV), NS, prm.stride, prm.cap, 0).total);
+    const size_t lds = static_cast<size_t>(lds_layout(sizeof(TT), prm.R, prm.C, prm.cap, 0).total);
     const unsigned grid = static_cast<unsigned>(prm.ntiles < 0x7fffffff ? prm.ntiles : 0x7fffffff);
     if (grid == 0)
       return;

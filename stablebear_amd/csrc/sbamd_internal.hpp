@@ -171,7 +171,7 @@ namespace sbamd
     int32_t out_kind = SBAMD_OUT_DISTMAT;
     int32_t precision = SBAMD_PREC_FP64;
     int32_t tt_f64 = 1, tv_f64 = 1;
-    int32_t R = 16, C = 16, cap = 64;
+    int32_t R = 16, C = 16, cap = 48; // measured best on MI355X for m ~ 50..500
     int64_t fold_threshold = 8192;
     uint64_t pairs = 0, segments = 0, out_count = 0;
     std::vector<Problem> h_probs;

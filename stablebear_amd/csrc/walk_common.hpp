@@ -138,33 +138,51 @@ namespace sbamd
     return hi * (hi + 1) / 2 + lo;
   }
 
-  // Byte layout of the dynamic LDS block of a tile kernel.
+  // One staged record in LDS: end time and value of a constant piece. 16 bytes
+  // for FP64 times (so one record is one aligned 16-byte access), 8 for FP32.
+  template <typename TT, typename TV>
+  struct alignas(sizeof(TT) == 8 ? 16 : 8) Rec
+  {
+    TT t;
+    TV v;
+  };
+
+  // Byte layout of the dynamic LDS block of a tile kernel:
+  //   slot s (rows 0..R-1, then columns), record j at
+  //     rows_off + (s * slot_recs(cap) + j) * REC
+  //   Every slot is contiguous with the same record stride, so advancing a
+  //   walk pointer is one add whichever side it belongs to; slot_recs is 1
+  //   modulo 16 so 16-byte records of different slots at the same position,
+  //   and of one slot at nearby positions, fall in different bank groups.
+  //   Then per-slot tables, the phase end time, and optional per-record extra
+  //   space (backward gradient slots) mirroring the record geometry.
+  constexpr int ROW_PAD = 4;
+
+  __host__ __device__ inline int slot_recs(int cap) { return (cap + 15) / 16 * 16 + 1; }
+
   struct LdsLayout
   {
-    int tend_off, val_off, base_off, m_off, cur_off, next_off, tstart_off, tb_off, extra_off, total;
+    int rec_bytes;
+    int rows_off, cols_off, rec_end;
+    int base_off, m_off, cur_off, next_off, tstart_off, lbase_off, lstride_off, tb_off;
+    int extra_off, total;
   };
 
   __host__ __device__ inline int align_up(int x, int a) { return (x + a - 1) / a * a; }
 
-  // Workgroup size of a tile kernel: one lane per pair, whole wavefronts.
-  inline unsigned tile_block_threads(int R, int C)
-  {
-    const int lanes = R * C;
-    return static_cast<unsigned>(lanes <= 64 ? 64 : (lanes + 63) / 64 * 64);
-  }
-
-  // extra_per_slot_rec: bytes per staged record for backward gradient slots.
-  __host__ __device__ inline LdsLayout lds_layout(int tt_bytes, int tv_bytes, int NS, int stride, int cap,
-                                                  int extra_per_rec)
+  // extra_per_rec: bytes per staged record of the optional extra region
+  // (a multiple of rec_bytes; placed at extra_off + rec_offset * extra_per_rec / rec_bytes).
+  __host__ __device__ inline LdsLayout lds_layout(int tt_bytes, int R, int C, int cap, int extra_per_rec)
   {
     LdsLayout L;
+    const int NS = R + C;
+    L.rec_bytes = tt_bytes == 8 ? 16 : 8;
     int o = 0;
-    L.tend_off = o;
-    o += tt_bytes * cap * stride;
-    o = align_up(o, 8);
-    L.val_off = o;
-    o += tv_bytes * cap * stride;
-    o = align_up(o, 8);
+    L.rows_off = o;
+    o += R * slot_recs(cap) * L.rec_bytes;
+    L.cols_off = o;
+    o += C * slot_recs(cap) * L.rec_bytes;
+    L.rec_end = o;
     L.base_off = o;
     o += 8 * NS;
     L.m_off = o;
@@ -175,12 +193,169 @@ namespace sbamd
     o += 8 * NS;
     L.tstart_off = o;
     o += 8 * NS;
+    L.lbase_off = o;
+    o += 4 * NS;
+    L.lstride_off = o;
+    o += 4 * NS;
+    o = align_up(o, 8);
     L.tb_off = o;
     o += 16;
-    o = align_up(o, 8);
+    o = align_up(o, 16);
     L.extra_off = o;
-    o += extra_per_rec * cap * stride;
+    o += (L.rec_end / L.rec_bytes) * extra_per_rec;
     L.total = align_up(o, 16);
     return L;
+  }
+
+  // Workgroup size of a tile kernel: one lane per pair, whole wavefronts.
+  inline unsigned tile_block_threads(int R, int C)
+  {
+    const int lanes = R * C;
+    return static_cast<unsigned>(lanes <= 64 ? 64 : (lanes + 63) / 64 * 64);
+  }
+
+  // ---------------------------------------------------------------------
+  // Phase staging shared by the forward and backward tile kernels.
+  // ---------------------------------------------------------------------
+  template <typename TT, typename TV>
+  struct TileLds
+  {
+    unsigned char* smem;
+    LdsLayout lay;
+    int64_t* base;   // global record offset of each slot's PCF
+    int64_t* m;      // records of each slot's PCF (0 for an unused slot)
+    int64_t* cur;    // first record staged this phase
+    int64_t* next;   // first record starting at or after this phase's end
+    TT* tstart;      // start time of record `next` (phase bookkeeping)
+    uint32_t* lbase; // LDS byte address of record 0 of the slot's phase window
+    uint32_t* lstride;
+    typename Lim<TT>::Bits* tb;
+
+    __device__ TileLds(unsigned char* s, const LdsLayout& L) : smem(s), lay(L)
+    {
+      base = reinterpret_cast<int64_t*>(s + L.base_off);
+      m = reinterpret_cast<int64_t*>(s + L.m_off);
+      cur = reinterpret_cast<int64_t*>(s + L.cur_off);
+      next = reinterpret_cast<int64_t*>(s + L.next_off);
+      tstart = reinterpret_cast<TT*>(s + L.tstart_off);
+      lbase = reinterpret_cast<uint32_t*>(s + L.lbase_off);
+      lstride = reinterpret_cast<uint32_t*>(s + L.lstride_off);
+      tb = reinterpret_cast<typename Lim<TT>::Bits*>(s + L.tb_off);
+    }
+
+    __device__ __forceinline__ Rec<TT, TV>& rec(uint32_t addr) const
+    {
+      return *reinterpret_cast<Rec<TT, TV>*>(smem + addr);
+    }
+  };
+
+  // Tile start: slot s < R is row PCF r0 + s, slot R + c is column c0 + c.
+  template <typename TT, typename TV>
+  __device__ void tile_begin(const TileLds<TT, TV>& S, const FwdParams& P, const Tile& tl,
+                             const Problem& pr, int tid)
+  {
+    using L = Lim<TT>;
+    const int NS = P.R + P.C;
+    if (tid < NS)
+    {
+      const bool isrow = tid < P.R;
+      const int k = isrow ? tid : tid - P.R;
+      const int lim = isrow ? tl.nr : tl.nc;
+      const int64_t* off = isrow ? P.r_off : P.c_off;
+      if (k < lim)
+      {
+        const int64_t g = isrow ? pr.row_base + tl.r0 + k : pr.col_base + tl.c0 + k;
+        const int64_t b = off[g];
+        S.base[tid] = b;
+        S.m[tid] = off[g + 1] - b;
+      }
+      else
+      {
+        S.base[tid] = 0;
+        S.m[tid] = 0;
+      }
+      S.next[tid] = 0;
+      S.tstart[tid] = TT(0);
+      const uint32_t rb = static_cast<uint32_t>(S.lay.rec_bytes);
+      S.lbase[tid] = static_cast<uint32_t>(S.lay.rows_off) +
+                     static_cast<uint32_t>(tid * slot_recs(P.cap)) * rb;
+      S.lstride[tid] = rb;
+    }
+    if (tid == 0)
+    {
+      S.tb[0] = L::bits(L::inf());
+      S.tb[1] = L::bits(L::inf());
+    }
+  }
+
+  // First half of a phase (before the barrier): each slot proposes the phase
+  // end (start time of its (cap+1)-th unconsumed record), and the workgroup
+  // copies up to cap records of every slot into LDS.
+  template <typename TT, typename TV>
+  __device__ void phase_stage(const TileLds<TT, TV>& S, const FwdParams& P, int phase, int tid,
+                              int nthreads)
+  {
+    using L = Lim<TT>;
+    const int NS = P.R + P.C;
+    const int cap = P.cap;
+    const TT* __restrict__ g_rt = static_cast<const TT*>(P.r_tend);
+    const TV* __restrict__ g_rv = static_cast<const TV*>(P.r_val);
+    const TT* __restrict__ g_ct = static_cast<const TT*>(P.c_tend);
+    const TV* __restrict__ g_cv = static_cast<const TV*>(P.c_val);
+    if (tid < NS)
+    {
+      const int64_t c = S.next[tid];
+      S.cur[tid] = c;
+      const int64_t rem = S.m[tid] - c;
+      TT tbs = L::inf();
+      if (rem > cap)
+        tbs = (tid < P.R ? g_rt : g_ct)[S.base[tid] + c + cap - 1];
+      atomicMin(&S.tb[phase & 1], L::bits(tbs));
+    }
+    for (int idx = tid; idx < NS * cap; idx += nthreads)
+    {
+      const int s = idx / cap;
+      const int j = idx - s * cap;
+      const int64_t k = S.next[s] + j;
+      if (k < S.m[s])
+      {
+        const bool isrow = s < P.R;
+        const int64_t gk = S.base[s] + k;
+        Rec<TT, TV> r;
+        r.t = isrow ? g_rt[gk] : g_ct[gk];
+        r.v = isrow ? g_rv[gk] : g_cv[gk];
+        S.rec(S.lbase[s] + static_cast<uint32_t>(j) * S.lstride[s]) = r;
+      }
+    }
+    if (tid == 0)
+      S.tb[(phase + 1) & 1] = L::bits(L::inf());
+  }
+
+  // After the barrier: the slot threads find the first staged record that
+  // starts at or after Tb (binary search) -- the next phase's cursor.
+  template <typename TT, typename TV>
+  __device__ void phase_advance_cursor(const TileLds<TT, TV>& S, const FwdParams& P, TT Tb, int tid)
+  {
+    const int NS = P.R + P.C;
+    if (tid < NS)
+    {
+      const int64_t c = S.cur[tid];
+      const int64_t rem = S.m[tid] - c;
+      const int lim = rem < P.cap ? static_cast<int>(rem) : P.cap;
+      const uint32_t lb = S.lbase[tid], ls = S.lstride[tid];
+      int lo = 0, hi = lim;
+      while (lo < hi)
+      {
+        const int mid = (lo + hi) >> 1;
+        const TT ts = mid == 0 ? S.tstart[tid] : S.rec(lb + static_cast<uint32_t>(mid - 1) * ls).t;
+        if (ts < Tb)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      S.next[tid] = c + lo;
+      if (lo > 0)
+        S.tstart[tid] = S.rec(lb + static_cast<uint32_t>(lo - 1) * ls).t;
+    }
   }
 }
