@@ -144,6 +144,11 @@ int sbamd_plan_out_offset(const sbamd_plan* plan, int32_t problem, uint64_t* off
 /* Pairs whose union walk spans at least this many segments fold FP32C
  * partials into FP64 per phase (default 8192). */
 int sbamd_plan_set_fold_threshold(sbamd_plan* plan, int64_t segments);
+/* Persistent launch (on by default for batched plans): as many workgroups as
+ * fit on the device take tiles, longest first, from an atomic counter that a
+ * fill kernel resets before each launch (graph-capture safe). Off: one
+ * workgroup per tile. */
+int sbamd_plan_set_persistent(sbamd_plan* plan, int32_t on);
 
 /* Enqueue the whole forward computation on `stream` (a hipStream_t; NULL is
  * the null stream, as in HIP). out_device: device buffer of plan out_count elements of
@@ -153,6 +158,22 @@ int sbamd_plan_set_fold_threshold(sbamd_plan* plan, int64_t segments);
  * (cuda_pcf_kernel.cuh:95-144, cuda_result_writer.hpp:46-108). */
 int sbamd_plan_execute(sbamd_plan* plan, void* out_device, int32_t out_dtype, void* row_sum_device,
                        void* stream);
+
+/* Synchronous convenience: run the plan into a library-owned device buffer
+ * and copy the result to out_host (plan out_count elements of out_dtype). */
+int sbamd_plan_execute_to_host(sbamd_plan* plan, void* out_host, int32_t out_dtype);
+
+/* The plain FP64 GPU kernel: one lane per pair walking the raw breakpoints
+ * of the two PCFs in global memory, in the original PCF order, with the
+ * reference walk (iterate_rectangles.hpp:37-104) and integral
+ * (matrix_integrate.hpp:18-29). The same design as the reference's CUDA
+ * kernel (cuda_pcf_kernel.cuh:95-136), kept as the measured GPU baseline
+ * beside the tiled kernels. FP64 sets only; out_kind DISTMAT or SYMMAT. */
+typedef struct sbamd_plain sbamd_plain;
+int sbamd_plain_create(sbamd_ctx* ctx, const sbamd_pcf_set* set, sbamd_plain** out);
+int sbamd_plain_execute(sbamd_plain* plain, int32_t op, double p, int32_t out_kind, void* out_device,
+                        void* stream);
+int sbamd_plain_destroy(sbamd_plain* plain);
 
 /* Backward of L = sum_e grad_out[e] * out[e] over the plan's output entries
  * (no reference counterpart). op must be SBAMD_OP_LP with p == 2 (fwd_out =

@@ -14,6 +14,7 @@ an autotuner for pair-tile shapes.
 """
 
 from . import system
+from .batched import l2_kernel_batched, pdist_batched
 from .distance import Distance_f32_f32, Distance_f64_f64, cdist, lp_distance, pdist
 from .distance_matrix import DistanceMatrix
 from .inner_product import InnerProduct_f32_f32, InnerProduct_f64_f64, l2_kernel
@@ -28,6 +29,7 @@ __all__ = [
     "Pcf", "PcfTensor", "IntPcfTensor", "FloatTensor", "PcfContainerLike", "zeros",
     "DistanceMatrix", "SymmetricMatrix", "Rectangle",
     "pdist", "cdist", "l2_kernel", "lp_distance", "iterate_rectangles",
+    "pdist_batched", "l2_kernel_batched",
     "Distance_f32_f32", "Distance_f64_f64", "InnerProduct_f32_f32", "InnerProduct_f64_f64",
     "dtype", "float32", "float64", "int32", "int64", "pcf32", "pcf64", "pcf32i", "pcf64i",
     "system",

@@ -81,8 +81,13 @@ _SIGS = {
     "sbamd_plan_info": (None, [vp, ctypes.POINTER(PlanStats)]),
     "sbamd_plan_out_offset": (None, [vp, ctypes.c_int32, c_u64p]),
     "sbamd_plan_set_fold_threshold": (None, [vp, ctypes.c_int64]),
+    "sbamd_plan_set_persistent": (None, [vp, ctypes.c_int32]),
     "sbamd_plan_execute": (None, [vp, vp, ctypes.c_int32, vp, vp]),
     "sbamd_plan_backward": (None, [vp, vp, vp, ctypes.c_int32, vp, vp, ctypes.c_int32, vp]),
+    "sbamd_plan_execute_to_host": (None, [vp, vp, ctypes.c_int32]),
+    "sbamd_plain_create": (None, [vp, ctypes.POINTER(PcfSet), ctypes.POINTER(vp)]),
+    "sbamd_plain_execute": (None, [vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, vp, vp]),
+    "sbamd_plain_destroy": (None, [vp]),
     "sbamd_graph_create": (None, [vp, vp, ctypes.c_int32, ctypes.POINTER(vp)]),
     "sbamd_graph_launch": (None, [vp, vp]),
     "sbamd_graph_destroy": (None, [vp]),

@@ -215,6 +215,8 @@ namespace sbamd
       use_device(pl.ctx);
       upload(pl.probs, pl.h_probs);
       upload(pl.tiles, pl.h_tiles);
+      if (!pl.counter.ptr)
+        pl.counter = DeviceBuffer(16);
     }
 
     FwdParams fwd_params(const Plan& pl, void* out, int32_t out_dtype, double* row_sum)
@@ -271,6 +273,12 @@ namespace sbamd
       FwdParams P = fwd_params(pl, out, out_dtype, row_sum);
       P.tiles = pl.tiles.as<Tile>() + t0;
       P.ntiles = t1 - t0;
+      if (pl.persistent && P.ntiles > 0)
+      {
+        // reset by a kernel, not hipMemsetAsync, so graph replays stay correct
+        launch_fill_zero(pl.counter.ptr, pl.counter.bytes, stream);
+        P.work_counter = pl.counter.as<uint32_t>();
+      }
       launch_forward(P, pl.tt_f64, pl.tv_f64, kernel_op(pl.op, pl.p),
                      pl.precision == SBAMD_PREC_FP32C ? 1 : 0, stream);
     }
@@ -356,6 +364,13 @@ struct sbamd_graph
   Ctx* ctx = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+};
+
+struct sbamd_plain
+{
+  Ctx* ctx = nullptr;
+  int64_t n = 0;
+  DeviceBuffer off, t, v;
 };
 
 extern "C"
@@ -584,6 +599,7 @@ extern "C"
       auto pl = std::make_unique<sbamd_plan>();
       pl->ctx = ctx;
       pl->batched = true;
+      pl->persistent = true;
       pl->op = op;
       pl->p = op == SBAMD_OP_LP ? p : 1.0;
       pl->out_kind = out_kind;
@@ -704,6 +720,14 @@ extern "C"
     });
   }
 
+  int sbamd_plan_set_persistent(sbamd_plan* plan, int32_t on)
+  {
+    return guarded([&] {
+      need(plan, "plan");
+      plan->persistent = on != 0;
+    });
+  }
+
   int sbamd_plan_set_shape(sbamd_plan* plan, const sbamd_tile_shape* shape)
   {
     return guarded([&] {
@@ -738,6 +762,78 @@ extern "C"
         launch_fill_zero(row_sum_device, static_cast<size_t>(plan->sets[0]->n) * sizeof(double), st);
       execute_range(*plan, out_device, out_dtype, static_cast<double*>(row_sum_device), 0,
                     static_cast<int64_t>(plan->h_tiles.size()), st);
+    });
+  }
+
+  int sbamd_plan_execute_to_host(sbamd_plan* plan, void* out_host, int32_t out_dtype)
+  {
+    return guarded([&] {
+      need(plan, "plan");
+      check_out_dtype(out_dtype);
+      if (plan->out_count == 0)
+        return;
+      need(out_host, "out_host");
+      use_device(plan->ctx);
+      const size_t eb = out_dtype == SBAMD_F64 ? 8 : 4;
+      DeviceBuffer dout(static_cast<size_t>(plan->out_count) * eb);
+      execute_range(*plan, dout.ptr, out_dtype, nullptr, 0, static_cast<int64_t>(plan->h_tiles.size()),
+                    plan->ctx->stream);
+      SBAMD_HIP(hipStreamSynchronize(plan->ctx->stream));
+      SBAMD_HIP(hipMemcpy(out_host, dout.ptr, static_cast<size_t>(plan->out_count) * eb,
+                          hipMemcpyDeviceToHost));
+    });
+  }
+
+  int sbamd_plain_create(sbamd_ctx* ctx, const sbamd_pcf_set* set, sbamd_plain** out)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      need(set, "set");
+      need(out, "out");
+      *out = nullptr;
+      if (set->dtype != SBAMD_F64)
+        fail(SBAMD_ERR_INVALID, "the plain FP64 kernel takes float64 PCF sets");
+      validate_set(*set);
+      auto pl = std::make_unique<sbamd_plain>();
+      pl->ctx = ctx;
+      pl->n = set->n;
+      use_device(ctx);
+      const int64_t M = set->n > 0 ? set->offsets[set->n] : 0;
+      std::vector<int64_t> off(set->offsets, set->offsets + set->n + 1);
+      std::vector<double> t(static_cast<const double*>(set->t), static_cast<const double*>(set->t) + M);
+      std::vector<double> v(static_cast<const double*>(set->v), static_cast<const double*>(set->v) + M);
+      upload(pl->off, off);
+      upload(pl->t, t);
+      upload(pl->v, v);
+      *out = pl.release();
+    });
+  }
+
+  int sbamd_plain_execute(sbamd_plain* plain, int32_t op, double p, int32_t out_kind, void* out_device,
+                          void* stream)
+  {
+    return guarded([&] {
+      need(plain, "plain");
+      if (op == SBAMD_OP_LP && !(p >= 1.0))
+        fail(SBAMD_ERR_INVALID, "p must be >= 1.");
+      if (out_kind != SBAMD_OUT_DISTMAT && out_kind != SBAMD_OUT_SYMMAT)
+        fail(SBAMD_ERR_INVALID, "the plain kernel writes DISTMAT or SYMMAT");
+      if (plain->n > 1 || (plain->n == 1 && out_kind == SBAMD_OUT_SYMMAT))
+        need(out_device, "out_device");
+      use_device(plain->ctx);
+      launch_plain(plain->off.as<int64_t>(), plain->t.as<double>(), plain->v.as<double>(), plain->n,
+                   kernel_op(op, p), op == SBAMD_OP_LP ? p : 1.0, out_kind,
+                   static_cast<double*>(out_device), pick_stream(plain->ctx, stream));
+    });
+  }
+
+  int sbamd_plain_destroy(sbamd_plain* plain)
+  {
+    return guarded([&] {
+      if (!plain)
+        return;
+      use_device(plain->ctx);
+      delete plain;
     });
   }
 

@@ -33,8 +33,22 @@ namespace sbamd
     const int nthreads = blockDim.x;
     const TV p = static_cast<TV>(P.p);
 
-    for (int64_t ti = blockIdx.x; ti < P.ntiles; ti += gridDim.x)
+    // Tiles come either one per workgroup (grid = tiles, the dispatcher hands
+    // them out in launch order) or, for persistent workgroups, from an atomic
+    // counter over the longest-first tile list.
+    uint32_t* const s_next_tile = reinterpret_cast<uint32_t*>(smem + lay.misc_off);
+    int64_t ti = blockIdx.x;
+    for (;; ti += gridDim.x)
     {
+      if (P.work_counter)
+      {
+        if (tid == 0)
+          *s_next_tile = atomicAdd(P.work_counter, 1u);
+        __syncthreads();
+        ti = *s_next_tile;
+      }
+      if (ti >= P.ntiles)
+        break;
       const Tile tl = P.tiles[ti];
       const Problem pr = P.probs[tl.prob];
       const bool fold = (tl.flags & TILE_FOLD) != 0;
@@ -168,25 +182,39 @@ namespace sbamd
   static void launch_typed(const FwdParams& prm, int op, hipStream_t stream)
   {
     const size_t lds = static_cast<size_t>(lds_layout(sizeof(TT), prm.R, prm.C, prm.cap, 0).total);
-    const unsigned grid = static_cast<unsigned>(prm.ntiles < 0x7fffffff ? prm.ntiles : 0x7fffffff);
-    if (grid == 0)
+    if (prm.ntiles <= 0)
       return;
-    const dim3 block(tile_block_threads(prm.R, prm.C));
+    const unsigned threads = tile_block_threads(prm.R, prm.C);
+    void (*kern)(FwdParams) = nullptr;
     switch (op)
     {
     case OPK_L1:
-      hipLaunchKernelGGL((fwd_tile_kernel<TT, TV, OPK_L1, KAHAN>), dim3(grid), block, lds, stream, prm);
+      kern = fwd_tile_kernel<TT, TV, OPK_L1, KAHAN>;
       break;
     case OPK_L2:
-      hipLaunchKernelGGL((fwd_tile_kernel<TT, TV, OPK_L2, KAHAN>), dim3(grid), block, lds, stream, prm);
+      kern = fwd_tile_kernel<TT, TV, OPK_L2, KAHAN>;
       break;
     case OPK_LP:
-      hipLaunchKernelGGL((fwd_tile_kernel<TT, TV, OPK_LP, KAHAN>), dim3(grid), block, lds, stream, prm);
+      kern = fwd_tile_kernel<TT, TV, OPK_LP, KAHAN>;
       break;
     default:
-      hipLaunchKernelGGL((fwd_tile_kernel<TT, TV, OPK_DOT, KAHAN>), dim3(grid), block, lds, stream, prm);
+      kern = fwd_tile_kernel<TT, TV, OPK_DOT, KAHAN>;
       break;
     }
+    int64_t grid = prm.ntiles;
+    if (prm.work_counter)
+    {
+      // persistent: as many workgroups as can be resident at once
+      int per_cu = 0, dev = 0, cus = 0;
+      SBAMD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, static_cast<int>(threads), lds));
+      SBAMD_HIP(hipGetDevice(&dev));
+      SBAMD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      const int64_t resident = static_cast<int64_t>(per_cu > 0 ? per_cu : 1) * cus;
+      grid = grid < resident ? grid : resident;
+    }
+    if (grid > 0x7fffffff)
+      grid = 0x7fffffff;
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(threads), lds, stream, prm);
     SBAMD_HIP(hipGetLastError());
   }
 

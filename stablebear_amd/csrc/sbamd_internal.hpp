@@ -84,6 +84,7 @@ namespace sbamd
     int32_t R, C, cap, stride;
     int32_t symmetric; // rows and cols are the same set (pdist / kernel)
     int32_t include_diag;
+    uint32_t* work_counter; // non-null: persistent workgroups take tiles from this counter
   };
 
   struct BwdParams
@@ -179,6 +180,9 @@ namespace sbamd
     DeviceBuffer probs, tiles;
     // batched: concatenated set arrays
     DeviceBuffer b_roff, b_tend, b_val, b_perm;
+    // persistent launch: resident workgroups pull tiles from this counter
+    bool persistent = false;
+    DeviceBuffer counter;
     uint64_t input_bytes = 0;
     void rebuild_tiles();
   };
@@ -198,6 +202,8 @@ namespace sbamd
                              int64_t cap, int64_t* count, hipStream_t stream);
   void launch_update_values(const void* v, const int64_t* src, int64_t records, void* rec_val,
                             int in_f64, int out_f64, hipStream_t stream);
+  void launch_plain(const int64_t* off, const double* t, const double* v, int64_t n, int op,
+                    double p, int out_kind, double* out, hipStream_t stream);
 
   // Planning
   void build_tiles(const std::vector<int64_t>& m_rows_sorted, const std::vector<int64_t>* m_cols_sorted,

@@ -164,7 +164,7 @@ namespace sbamd
   {
     int rec_bytes;
     int rows_off, cols_off, rec_end;
-    int base_off, m_off, cur_off, next_off, tstart_off, lbase_off, lstride_off, tb_off;
+    int base_off, m_off, cur_off, next_off, tstart_off, lbase_off, lstride_off, tb_off, misc_off;
     int extra_off, total;
   };
 
@@ -199,6 +199,8 @@ namespace sbamd
     o += 4 * NS;
     o = align_up(o, 8);
     L.tb_off = o;
+    o += 16;
+    L.misc_off = o; // next tile index of a persistent workgroup
     o += 16;
     o = align_up(o, 16);
     L.extra_off = o;

@@ -327,6 +327,9 @@ class Plan:
     def set_fold_threshold(self, segments: int):
         L.check(L.lib().sbamd_plan_set_fold_threshold(self.handle, int(segments)))
 
+    def set_persistent(self, on: bool):
+        L.check(L.lib().sbamd_plan_set_persistent(self.handle, 1 if on else 0))
+
     def set_shape(self, rows, cols, cap):
         sh = L.TileShape(rows, cols, cap, 0)
         L.check(L.lib().sbamd_plan_set_shape(self.handle, ctypes.byref(sh)))
@@ -336,6 +339,12 @@ class Plan:
                 stream: int | None = None):
         L.check(L.lib().sbamd_plan_execute(self.handle, out_ptr, L.F64 if out_f64 else L.F32,
                                            row_sum_ptr, stream))
+
+    def execute_to_host(self, out_f64: bool) -> np.ndarray:
+        out = np.empty(self.out_count, dtype=np.float64 if out_f64 else np.float32)
+        L.check(L.lib().sbamd_plan_execute_to_host(self.handle, out.ctypes.data if out.size else None,
+                                                   L.F64 if out_f64 else L.F32))
+        return out
 
     def backward(self, grad_out_ptr, fwd_out_ptr, out_f64, grad_v_ptr, grad_t_ptr, grad_f64,
                  stream=None):
@@ -382,6 +391,35 @@ class Graph:
     def close(self):
         if getattr(self, "handle", None):
             L.lib().sbamd_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PlainKernel:
+    """The plain FP64 GPU kernel (one lane per pair over raw breakpoints in
+    global memory): the measured GPU baseline of the tiled kernels."""
+
+    def __init__(self, csr: Csr, device: int | None = None):
+        self.ctx = context(device)
+        self.csr = csr
+        self.n = csr.n
+        s = csr.struct()
+        h = ctypes.c_void_p()
+        L.check(L.lib().sbamd_plain_create(self.ctx, ctypes.byref(s), ctypes.byref(h)))
+        self.handle = h
+
+    def execute(self, op: str, p: float, out_kind: int, out_ptr: int, stream: int | None = None):
+        L.check(L.lib().sbamd_plain_execute(self.handle, OPS[op], float(p), out_kind, out_ptr,
+                                            stream))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            L.lib().sbamd_plain_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

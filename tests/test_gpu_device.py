@@ -1,0 +1,190 @@
+"""GPU tests of the device-level C-ABI paths: device-resident sets and plans,
+tile shapes and the autotuner, persistent launches, hipGraph replays, the
+batched launch, the plain FP64 baseline kernel, cdist row sums, and the
+asynchronous job protocol (progress and cancellation)."""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import stablebear_amd as sb
+from stablebear_amd import _lib as L
+from stablebear_amd import runtime, workloads
+from stablebear_amd.batched import BatchedPlan
+
+pytestmark = pytest.mark.gpu
+
+
+def _plan(off, t, v, prec="fp64", op="l1", p=1.0, shape=None):
+    ds = runtime.DeviceSet(runtime.csr_from_arrays(off, t, v), prec)
+    return ds, runtime.Plan(ds, op=op, p=p, shape=shape)
+
+
+def _run(plan, f64):
+    out = torch.empty(plan.out_count, dtype=torch.float64 if f64 else torch.float32, device="cuda")
+    plan.execute(out.data_ptr(), f64, stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 24), (16, 16, 48), (8, 8, 32), (16, 16, 2), (4, 4, 7)])
+def test_tile_shapes_all_bit_exact(gpu, shape):
+    off, t, v = workloads.c2(90, seed=3)
+    exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_PDIST, off, t, v)
+    ds, pl = _plan(off, t, v, shape=shape)
+    assert pl.shape == shape
+    np.testing.assert_array_equal(_run(pl, True), exp)
+
+
+def test_persistent_launch_matches(gpu):
+    off, t, v = workloads.c2(300, seed=4)
+    exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_PDIST, off, t, v)
+    ds, pl = _plan(off, t, v)
+    pl.set_persistent(True)
+    np.testing.assert_array_equal(_run(pl, True), exp)
+    pl.set_persistent(False)
+    np.testing.assert_array_equal(_run(pl, True), exp)
+
+
+def test_execute_to_host_and_fold_threshold(gpu):
+    off, t, v = workloads.c2(120, seed=5)
+    exp = oracle.pairwise(oracle.OP_LP, 2.0, oracle.KIND_PDIST, off, t, v)
+    ds, pl = _plan(off, t, v, prec="fp32c", op="lp", p=2.0)
+    got = pl.execute_to_host(False).astype(np.float64)
+    assert np.max(np.abs(got - exp) / exp) <= 1e-6
+    pl.set_fold_threshold(2)  # fold every pair's partial sums into FP64 each phase
+    got2 = pl.execute_to_host(False).astype(np.float64)
+    assert np.max(np.abs(got2 - exp) / exp) <= 1e-6
+
+
+def test_autotune_picks_a_valid_shape(gpu):
+    off, t, v = workloads.c2(400, seed=6)
+    exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_PDIST, off, t, v)
+    ds, pl = _plan(off, t, v, prec="fp32c")
+    out = torch.empty(pl.out_count, dtype=torch.float32, device="cuda")
+    shape, ms = pl.autotune(out.data_ptr(), False, reps=1)
+    assert ms > 0 and pl.shape == shape
+    got = _run(pl, False).astype(np.float64)
+    assert np.max(np.abs(got - exp) / exp) <= 1e-6
+
+
+def test_graph_replays_match_eager(gpu):
+    off, t, v = workloads.c2(250, seed=8)
+    ds, pl = _plan(off, t, v, prec="fp32c")
+    eager = _run(pl, False)
+    out = torch.zeros(pl.out_count, dtype=torch.float32, device="cuda")
+    g = pl.graph(out.data_ptr(), False)
+    for k in range(4):
+        out.fill_(-1.0)
+        torch.cuda.synchronize()
+        g.launch(torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), eager)
+        _ = torch.randn(1000, device="cuda").sum().item()  # eager work between replays
+
+
+def test_batched_persistent_graph_matches_oracle(gpu):
+    probs = [workloads.c3_problem(b, n=int(n)) for b, n in enumerate([70, 5, 133, 1, 64, 200, 2])]
+    bp = BatchedPlan(probs, op="l1", precision="fp32c")
+    assert bp.plan.problems == len(probs)
+    out = torch.zeros(bp.out_count, dtype=torch.float32, device="cuda")
+    g = bp.graph(out.data_ptr(), False)
+    for _ in range(3):
+        out.fill_(-7.0)
+        g.launch(torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        parts = bp.split(out.cpu().numpy())
+        for (off, t, v), got in zip(probs, parts):
+            exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_PDIST, off, t, v)
+            assert got.shape == exp.shape
+            if exp.size:
+                nz = exp != 0
+                assert np.max(np.abs(got[nz] - exp[nz]) / exp[nz]) <= 1e-6
+
+
+def test_batched_python_api_fp64_bit_exact(gpu):
+    probs = [workloads.c3_problem(b, n=n) for b, n in enumerate([40, 3, 77])]
+    Xs = [sb.PcfTensor(workloads.to_pcfs(*pr)) for pr in probs]
+    Ds = sb.pdist_batched(Xs)
+    Ks = sb.l2_kernel_batched(Xs)
+    for (off, t, v), D, K in zip(probs, Ds, Ks):
+        np.testing.assert_array_equal(D.storage, oracle.pairwise(oracle.OP_L1, 1.0, 0, off, t, v))
+        np.testing.assert_array_equal(K.storage, oracle.pairwise(oracle.OP_DOT, 1.0, 1, off, t, v))
+    D2 = sb.pdist_batched(Xs, p=2)
+    for (off, t, v), D in zip(probs, D2):
+        np.testing.assert_allclose(D.storage, oracle.pairwise(oracle.OP_LP, 2.0, 0, off, t, v),
+                                   rtol=1e-13)
+
+
+@pytest.mark.parametrize("op,kind,out_kind", [("l1", 0, L.OUT_DISTMAT), ("dot", 1, L.OUT_SYMMAT)])
+def test_plain_fp64_kernel_bit_exact(gpu, op, kind, out_kind):
+    off, t, v = workloads.c2(150, seed=9)
+    pk = runtime.PlainKernel(runtime.csr_from_arrays(off, t, v))
+    n = len(off) - 1
+    cnt = oracle.out_count(kind, n)
+    out = torch.empty(cnt, dtype=torch.float64, device="cuda")
+    pk.execute(op, 1.0, out_kind, out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    exp = oracle.pairwise(oracle.OP_L1 if op == "l1" else oracle.OP_DOT, 1.0, kind, off, t, v)
+    np.testing.assert_array_equal(out.cpu().numpy(), exp)
+
+
+def test_plain_kernel_on_duplicates_and_ties(gpu):
+    ref = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
+                                             "ref_vectors.npz"))
+    off, t, v = ref["edge_off"], ref["edge_t"], ref["edge_v"]
+    pk = runtime.PlainKernel(runtime.csr_from_arrays(off, t, v))
+    out = torch.empty(len(ref["edge_l1_f64"]), dtype=torch.float64, device="cuda")
+    pk.execute("l1", 1.0, L.OUT_DISTMAT, out.data_ptr(), None)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), ref["edge_l1_f64"])
+
+
+def test_cdist_row_sums(gpu):
+    X, Y = workloads.c5(60, m_lo=50, m_hi=400)
+    exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_CDIST, *X, *Y).reshape(60, 60)
+    sx = runtime.DeviceSet(runtime.csr_from_arrays(*X), "fp32c")
+    sy = runtime.DeviceSet(runtime.csr_from_arrays(*Y), "fp32c")
+    pl = runtime.Plan(sx, sy, op="l1")
+    out = torch.empty(pl.out_count, dtype=torch.float32, device="cuda")
+    rs = torch.full((60,), 123.0, dtype=torch.float64, device="cuda")
+    pl.execute(out.data_ptr(), False, rs.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    D = out.cpu().numpy().reshape(60, 60).astype(np.float64)
+    assert np.max(np.abs(D - exp) / exp) <= 1e-6
+    np.testing.assert_allclose(rs.cpu().numpy() / 60, exp.mean(axis=1), rtol=1e-6)
+
+
+def test_job_progress_and_cancel(gpu):
+    off, t, v = workloads.c2(3000, seed=10)
+    csr = runtime.csr_from_arrays(off, t, v)
+    n = 3000
+    out = np.zeros(n * (n - 1) // 2)
+    task = runtime.submit(csr, None, L.OP_L1, 1.0, L.OUT_DISTMAT, L.PREC_FP64, out, "d", len(out))
+    task.wait()
+    assert task.work_total() == len(out)
+    np.testing.assert_array_equal(out, oracle.pairwise(oracle.OP_L1, 1.0, 0, off, t, v))
+    out2 = np.zeros_like(out)
+    task2 = runtime.submit(csr, None, L.OP_L1, 1.0, L.OUT_DISTMAT, L.PREC_FP64, out2, "d", len(out))
+    task2.request_stop()
+    with pytest.raises(L.CancelledError):
+        task2.wait()
+
+
+def test_device_set_value_update(gpu):
+    off, t, v = workloads.c2(80, seed=12)
+    ds, pl = _plan(off, t, v)
+    v2 = v * 1.5 - 0.25
+    v2[off[1:] - 1] = 0.0
+    vd = torch.from_numpy(v2).cuda()
+    ds.update_values(vd.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    np.testing.assert_array_equal(_run(pl, True), oracle.pairwise(oracle.OP_L1, 1.0, 0, off, t, v2))
+
+
+def test_bad_inputs_raise(gpu):
+    with pytest.raises(ValueError, match="non-decreasing"):
+        runtime.DeviceSet(runtime.csr_from_arrays([0, 3], [0.0, 2.0, 1.0], [1.0, 1.0, 0.0]))
+    with pytest.raises(ValueError, match="p must be"):
+        off, t, v = workloads.c1(5)
+        _plan(off, t, v, op="lp", p=0.5)
