@@ -867,24 +867,50 @@ extern "C"
       const size_t gb = grad_dtype == SBAMD_F64 ? 8 : 4;
       launch_fill_zero(grad_v_device, static_cast<size_t>(s->points) * gb, st);
       launch_fill_zero(grad_t_device, static_cast<size_t>(s->points) * gb, st);
+      const size_t qbytes = static_cast<size_t>(s->records) * 8;
+      const size_t wbytes = static_cast<size_t>(s->n) * 8;
+      if (plan->bwd_q.bytes < qbytes || plan->bwd_w.bytes < wbytes)
+      {
+        plan->bwd_q = DeviceBuffer(qbytes);
+        plan->bwd_a = DeviceBuffer(qbytes);
+        plan->bwd_w = DeviceBuffer(wbytes);
+      }
+      launch_fill_zero(plan->bwd_q.ptr, qbytes, st);
+      launch_fill_zero(plan->bwd_a.ptr, qbytes, st);
+      launch_fill_zero(plan->bwd_w.ptr, wbytes, st);
       BwdParams B;
       std::memset(&B, 0, sizeof(B));
       B.f = fwd_params(*plan, nullptr, out_dtype, nullptr);
-      // the gradient slots double the LDS per staged record: shrink the phase
-      while (B.f.cap > 2 && bwd_lds_bytes(plan->tv_f64, B.f.R, B.f.C, B.f.cap) > plan->ctx->lds_per_block)
+      B.f.work_counter = nullptr;
+      // the accumulator slots double the LDS per staged record: a shorter
+      // phase keeps four workgroups resident per CU
+      B.f.cap = std::min(B.f.cap, plan->bwd_cap);      while (B.f.cap > 2 && bwd_lds_bytes(plan->tv_f64, B.f.R, B.f.C, B.f.cap) > plan->ctx->lds_per_block)
         B.f.cap /= 2;
       B.grad_out = grad_out_device;
       B.fwd_out = fwd_out_device;
-      B.r_src = s->src.as<int64_t>();
-      B.grad_v = grad_v_device;
-      B.grad_t = grad_t_device;
-      B.grad_f64 = grad_dtype == SBAMD_F64 ? 1 : 0;
+      B.q = plan->bwd_q.as<double>();
+      B.a = plan->bwd_a.as<double>();
+      B.wsum = plan->bwd_w.as<double>();
       B.in_f64 = out_dtype == SBAMD_F64 ? 1 : 0;
       B.val_f64 = plan->tv_f64;
-      for (int64_t r : s->h_src)
-        if (r < 0)
-          fail(SBAMD_ERR_INVALID, "backward needs sets without empty PCFs");
       launch_backward(B, kop, st);
+      FinalizeParams F;
+      std::memset(&F, 0, sizeof(F));
+      F.roff = s->roff.as<int64_t>();
+      F.tend = s->tend.as<double>();
+      F.val = s->val.ptr;
+      F.src = s->src.as<int64_t>();
+      F.q = B.q;
+      F.a = B.a;
+      F.wsum = B.wsum;
+      F.grad_v = grad_v_device;
+      F.grad_t = grad_t_device;
+      F.n = s->n;
+      F.records = s->records;
+      F.val_f64 = plan->tv_f64;
+      F.grad_f64 = grad_dtype == SBAMD_F64 ? 1 : 0;
+      F.op = kop;
+      launch_finalize_grad(F, st);
     });
   }
 

@@ -1,37 +1,32 @@
 // Backward of L = sum_e G_e * X_e over the pair outputs X of a pdist (p = 2)
 // or l2_kernel plan. No reference counterpart (the reference has no autograd).
 //
-// Per pair (f, g) with weight w (G / (2 D) for the L2 distance D = sqrt(S),
-// S = int (f - g)^2; G for the inner product K = int f g), walking the union
-// segments [t, tn) with values (fv, gv):
-//   dS/dv_f(record) += 2 (fv - gv) len,   dS/dv_g(record) -= 2 (fv - gv) len
-//   dK/dv_f(record) += gv len,            dK/dv_g(record) += fv len
-// and at each crossing of a breakpoint of f (the start time of its next
-// record, with the partner value c):
-//   dS/dt = (v_old - c)^2 - (v_new - c)^2,  dK/dt = (v_old - v_new) c
-// f advances first on ties, so f's breakpoint sees g's left-limit value and g's
-// breakpoint sees f's right-limit value; for a self pair (kernel diagonal) the
-// two halves add up to the exact derivative v_old^2 - v_new^2.
+// For a PCF f with records r (value v_r on [t_start(r), t_end(r))) and its
+// partners g_j with pair weights w_j (G / (2 D) for the L2 distance
+// D = sqrt(int (f - g)^2), G for the inner product K = int f g):
+//   dL/dv_r  (L2)  = 2 (v_r len_r W - Q_r)        (dot) = Q_r
+//   dL/dt_r  (L2)  = (v_{r-1} - v_r) ((v_{r-1} + v_r) W - 2 A_{r-1})
+//            (dot) = (v_{r-1} - v_r) A_{r-1}
+// with W = sum_j w_j, Q_r = sum_j w_j int_{record r} g_j, and A_r = sum_j w_j
+// g_j(t_end(r)) (the partner value where record r ends). The tile kernel
+// walks every pair once (the forward walk) accumulating Q and A per record:
+// w times the integral of the partner over the current record in a
+// register, added to the record's LDS slot together with w times the partner
+// value when the walk leaves the record; the slots go to global memory once
+// per phase. A small finalize kernel applies the formulas above.
 //
-// Value gradients of the current record accumulate in registers and are
-// flushed when the record is left; flushes and time gradients go to FP64 LDS
-// slots beside the staged records, flushed to global memory once per staged
-// record and phase. A record staged in an earlier phase is flushed straight
-// to global memory.
+// Ties: where f and g end a record at the same time, the side whose record
+// the walk already holds as the earlier-ending one leaves first and sees the
+// partner's left-limit value; the other side leaves second and sees the
+// first side's new, right-limit value. The derivative does not exist there;
+// for a self pair (kernel diagonal) the two halves add up to the exact
+// derivative whichever side leaves first.
 #include "walk_common.hpp"
 
 namespace sbamd
 {
-  __device__ __forceinline__ void gadd(void* base, int64_t idx, double x, int f64)
-  {
-    if (f64)
-      atomicAdd(static_cast<double*>(base) + idx, x);
-    else
-      atomicAdd(static_cast<float*>(base) + idx, static_cast<float>(x));
-  }
-
   template <typename TV, int OP>
-  __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdParams B)
+  __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void bwd_tile_kernel(BwdParams B)
   {
     extern __shared__ __align__(16) unsigned char smem[];
     using TT = double;
@@ -39,18 +34,18 @@ namespace sbamd
     using RecT = Rec<TT, TV>;
     const FwdParams& P = B.f;
 
-    // 16 bytes of gradient slots (value, time) per staged record
+    // 16 bytes of accumulators (Q, A) per staged record, then the W per slot
     const LdsLayout lay = lds_layout(sizeof(TT), P.R, P.C, P.cap, 16);
     const TileLds<TT, TV> S(smem, lay);
-    static_assert(sizeof(RecT) == 16, "gradient slots mirror 16-byte records");
-    constexpr uint32_t SF = sizeof(RecT); // record stride of every slot
-    constexpr uint32_t SG = SF;
+    static_assert(sizeof(RecT) == 16, "accumulator slots mirror 16-byte records");
+    constexpr uint32_t REC = sizeof(RecT);
     const uint32_t GOFF = static_cast<uint32_t>(lay.extra_off);
-    const int64_t* __restrict__ src = B.r_src;
+    double* const s_w = reinterpret_cast<double*>(smem + lay.total);
 
     const int tid = threadIdx.x;
     const int nthreads = blockDim.x;
     const int NS = P.R + P.C;
+    const uint32_t SR = static_cast<uint32_t>(slot_recs(P.cap));
 
     for (int64_t ti = blockIdx.x; ti < P.ntiles; ti += gridDim.x)
     {
@@ -58,15 +53,21 @@ namespace sbamd
       const Problem pr = P.probs[tl.prob];
 
       tile_begin(S, P, tl, pr, tid);
+      if (tid < NS)
+        s_w[tid] = 0.0;
       __syncthreads();
 
-      const int lr = tid / P.C;
-      const int lc = tid - lr * P.C;
+      // Lanes take the tile's pairs along diagonals, (r, (r + k) mod C) with
+      // r = tid mod R and k = tid / R, so a wavefront holds each row and each
+      // column only a few times: the lanes of one PCF that leave the same
+      // record in one instruction add to the same LDS word, and the
+      // hardware serialises those adds.
+      const int lr = tid % P.R;
+      const int lc = (lr + tid / P.R) % P.C;
       bool valid = tid < P.R * P.C && lr < tl.nr && lc < tl.nc;
       if (valid && (tl.flags & TILE_DIAG))
         valid = P.include_diag ? (lr >= lc) : (lr > lc);
 
-      // Pair weight.
       double w = 0.0;
       if (valid)
       {
@@ -90,110 +91,99 @@ namespace sbamd
       }
       const int slot_f = valid ? lr : 0;
       const int slot_g = valid ? P.R + lc : P.R;
-      const double scale = OP == OPK_DOT ? w : 2.0 * w;
+      if (valid)
+      {
+        atomicAdd(&s_w[slot_f], w);
+        atomicAdd(&s_w[slot_g], w);
+      }
 
+      // Walk state, values pre-multiplied by the pair weight w: lo is the
+      // side whose current record ends first (at lo_t), hi the other side.
       TT t = TT(0);
-      TT tf = L::inf(), tg = L::inf();
-      TV fv = TV(0), gv = TV(0);
-      int64_t nf = 1, ng = 1;
-      double accf = 0.0, accg = 0.0; // unscaled value-gradient sums of the current records
+      TT lo_t = L::inf(), hi_t = L::inf();
+      double lo_w = 0.0, hi_w = 0.0;     // w * current value of each side
+      double lo_acc = 0.0, hi_acc = 0.0; // w * integral of the partner over the current record
+      int lo_s = slot_f, hi_s = slot_g;
+      int64_t lo_k = 1, hi_k = 1; // record index of each side's next record
 
       for (int phase = 0;; ++phase)
       {
-        phase_stage(S, P, phase, tid, nthreads);
+        phase_stage<TT, TV, 1>(S, P, phase, tid, nthreads);
         for (int k = tid; k < lay.rec_end / 16; k += nthreads)
           *reinterpret_cast<double2*>(smem + GOFF + 16 * k) = make_double2(0.0, 0.0);
         __syncthreads();
         const TT Tb = L::from_bits(S.tb[phase & 1]);
-        phase_advance_cursor(S, P, Tb, tid);
+        phase_advance_cursor<TT, TV, 1>(S, P, Tb, tid);
 
         if (valid)
         {
-          const int64_t cf = S.cur[slot_f];
-          const int64_t cg = S.cur[slot_g];
-          const uint32_t bf = S.lbase[slot_f];
-          const uint32_t bg = S.lbase[slot_g];
-          const int64_t basef = S.base[slot_f];
-          const int64_t baseg = S.base[slot_g];
           if (phase == 0)
           {
-            const RecT rf = S.rec(bf);
-            const RecT rg = S.rec(bg);
-            tf = rf.t;
-            fv = rf.v;
-            tg = rg.t;
-            gv = rg.v;
+            // record 0 sits at local index 1 (index 0 is the lead slot)
+            const RecT rf = S.rec(S.lbase[slot_f] + REC);
+            const RecT rg = S.rec(S.lbase[slot_g] + REC);
+            const bool k = rf.t <= rg.t;
+            lo_t = k ? rf.t : rg.t;
+            lo_w = w * static_cast<double>(k ? rf.v : rg.v);
+            hi_t = k ? rg.t : rf.t;
+            hi_w = w * static_cast<double>(k ? rg.v : rf.v);
+            lo_s = k ? slot_f : slot_g;
+            hi_s = k ? slot_g : slot_f;
           }
-          uint32_t af_addr = bf + static_cast<uint32_t>(nf - cf) * SF;
-          uint32_t ag_addr = bg + static_cast<uint32_t>(ng - cg) * SG;
-          for (;;)
+          // window: record k at local k - cur + 1
+          uint32_t lo_p = S.lbase[lo_s] + static_cast<uint32_t>(lo_k - S.cur[lo_s] + 1) * REC;
+          uint32_t hi_p = S.lbase[hi_s] + static_cast<uint32_t>(hi_k - S.cur[hi_s] + 1) * REC;
+          while (lo_t < Tb)
           {
-            const bool af = tf <= tg;
-            const TT tn = af ? tf : tg;
-            if (!(tn < Tb))
-              break;
-            const double len = tn - t;
-            if constexpr (OP == OPK_DOT)
-            {
-              accf += static_cast<double>(gv) * len;
-              accg += static_cast<double>(fv) * len;
-            }
-            else
-            {
-              const double d = static_cast<double>(fv) - static_cast<double>(gv);
-              accf += d * len;
-              accg -= d * len;
-            }
-            t = tn;
-            const uint32_t a = af ? af_addr : ag_addr;
-            const uint32_t st = af ? SF : SG;
-            const uint32_t b0 = af ? bf : bg;
-            const RecT r = S.rec(a);
-            // flush the value gradient of the record being left
-            const double gflush = (af ? accf : accg) * scale;
-            if (a - b0 >= st)
-              atomicAdd(reinterpret_cast<double*>(smem + GOFF + a - st), gflush);
-            else
-              gadd(B.grad_v, src[(af ? basef + cf : baseg + cg) - 1], gflush, B.grad_f64);
-            // time gradient of the breakpoint just crossed (start of record r)
-            const double vold = af ? static_cast<double>(fv) : static_cast<double>(gv);
-            const double c = af ? static_cast<double>(gv) : static_cast<double>(fv);
-            const double vn = static_cast<double>(r.v);
-            double dt_grad;
-            if constexpr (OP == OPK_DOT)
-              dt_grad = (vold - vn) * c;
-            else
-              dt_grad = (vold - c) * (vold - c) - (vn - c) * (vn - c);
-            atomicAdd(reinterpret_cast<double*>(smem + GOFF + a + 8), dt_grad * w);
-            af_addr = af ? af_addr + SF : af_addr;
-            ag_addr = af ? ag_addr : ag_addr + SG;
-            tf = af ? r.t : tf;
-            fv = af ? r.v : fv;
-            tg = af ? tg : r.t;
-            gv = af ? gv : r.v;
-            accf = af ? 0.0 : accf;
-            accg = af ? accg : 0.0;
+            const double len = lo_t - t;
+            lo_acc = fma(hi_w, len, lo_acc);
+            hi_acc = fma(lo_w, len, hi_acc);
+            t = lo_t;
+            // lo leaves its current record (the one before lo_p)
+            double* const slot = reinterpret_cast<double*>(smem + GOFF + (lo_p - REC));
+            atomicAdd(slot, lo_acc);
+            atomicAdd(slot + 1, hi_w);
+            const RecT r = S.rec(lo_p);
+            const double rw = w * static_cast<double>(r.v);
+            const uint32_t np = lo_p + REC;
+            const bool k = r.t <= hi_t;
+            const TT nlt = k ? r.t : hi_t;
+            const double nlw = k ? rw : hi_w;
+            const uint32_t nlp = k ? np : hi_p;
+            const double nla = k ? 0.0 : hi_acc;
+            hi_t = k ? hi_t : r.t;
+            hi_w = k ? hi_w : rw;
+            hi_p = k ? hi_p : np;
+            hi_acc = k ? hi_acc : 0.0;
+            lo_t = nlt;
+            lo_w = nlw;
+            lo_p = nlp;
+            lo_acc = nla;
           }
-          nf = cf + (af_addr - bf) / SF;
-          ng = cg + (ag_addr - bg) / SG;
+          // a next-record pointer is at local 1 .. cap + 1 (slot_recs >= cap + 1),
+          // so local - 1 identifies the slot even at the end of a full window
+          const uint32_t lo_i = lo_p / REC, hi_i = hi_p / REC;
+          lo_s = static_cast<int>((lo_i - 1) / SR);
+          hi_s = static_cast<int>((hi_i - 1) / SR);
+          lo_k = S.cur[lo_s] - 1 + (lo_i - S.lbase[lo_s] / REC);
+          hi_k = S.cur[hi_s] - 1 + (hi_i - S.lbase[hi_s] / REC);
         }
         __syncthreads();
 
-        // flush this phase's record slots
-        for (int idx = tid; idx < NS * P.cap; idx += nthreads)
+        // this phase's accumulators: records cur - 1 .. next - 1 of each slot
+        for (int idx = tid; idx < NS * (P.cap + 1); idx += nthreads)
         {
-          const int s = idx / P.cap;
-          const int j = idx - s * P.cap;
-          const int64_t k = S.cur[s] + j;
-          if (k < S.next[s])
+          const int s = idx / (P.cap + 1);
+          const int j = idx - s * (P.cap + 1);
+          const int64_t k = S.cur[s] - 1 + j;
+          if (k >= 0 && k < S.next[s])
           {
-            const uint32_t a = S.lbase[s] + static_cast<uint32_t>(j) * S.lstride[s];
-            const double2 g2 = *reinterpret_cast<const double2*>(smem + GOFF + a);
-            const int64_t pt = src[S.base[s] + k];
-            if (g2.x != 0.0)
-              gadd(B.grad_v, pt, g2.x, B.grad_f64);
-            if (g2.y != 0.0 && k > 0)
-              gadd(B.grad_t, pt, g2.y, B.grad_f64);
+            const double2 qa = *reinterpret_cast<const double2*>(smem + GOFF + S.lbase[s] + j * REC);
+            const int64_t g = S.base[s] + k;
+            if (qa.x != 0.0)
+              atomicAdd(&B.q[g], qa.x);
+            if (qa.y != 0.0)
+              atomicAdd(&B.a[g], qa.y);
           }
         }
         __syncthreads();
@@ -203,35 +193,87 @@ namespace sbamd
 
       if (valid)
       {
-        // final rectangle [t, MAX) and flush of the last records
+        // final rectangle [t, MAX), then the last records' integrals
         const double len = L::max() - t;
-        if constexpr (OP == OPK_DOT)
-        {
-          if (gv != TV(0))
-            accf += static_cast<double>(gv) * len;
-          if (fv != TV(0))
-            accg += static_cast<double>(fv) * len;
-        }
-        else
-        {
-          const double d = static_cast<double>(fv) - static_cast<double>(gv);
-          if (d != 0.0)
-          {
-            accf += d * len;
-            accg -= d * len;
-          }
-        }
-        gadd(B.grad_v, src[S.base[slot_f] + nf - 1], accf * scale, B.grad_f64);
-        gadd(B.grad_v, src[S.base[slot_g] + ng - 1], accg * scale, B.grad_f64);
+        if (hi_w != 0.0)
+          lo_acc = fma(hi_w, len, lo_acc);
+        if (lo_w != 0.0)
+          hi_acc = fma(lo_w, len, hi_acc);
+        atomicAdd(&B.q[S.base[lo_s] + lo_k - 1], lo_acc);
+        atomicAdd(&B.q[S.base[hi_s] + hi_k - 1], hi_acc);
+      }
+      if (tid < NS)
+      {
+        const bool isrow = tid < P.R;
+        const int kk = isrow ? tid : tid - P.R;
+        if (kk < (isrow ? tl.nr : tl.nc) && s_w[tid] != 0.0)
+          atomicAdd(&B.wsum[isrow ? pr.row_base + tl.r0 + kk : pr.col_base + tl.c0 + kk], s_w[tid]);
       }
       __syncthreads();
+    }
+  }
+
+  // One thread per record: the formulas in the file comment.
+  __global__ void finalize_grad_kernel(FinalizeParams F)
+  {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= F.records)
+      return;
+    int64_t lo = 0, hi = F.n; // PCF q with roff[q] <= r < roff[q + 1]
+    while (hi - lo > 1)
+    {
+      const int64_t mid = (lo + hi) >> 1;
+      if (F.roff[mid] <= r)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const int64_t q = lo;
+    const int64_t first = F.roff[q];
+    auto val = [&](int64_t k) {
+      return F.val_f64 ? static_cast<const double*>(F.val)[k]
+                       : static_cast<double>(static_cast<const float*>(F.val)[k]);
+    };
+    const double v = val(r);
+    const double W = F.wsum[q];
+    double gv, gt = 0.0;
+    if (F.op == OPK_DOT)
+    {
+      gv = F.q[r];
+      if (r > first)
+        gt = (val(r - 1) - v) * F.a[r - 1];
+    }
+    else
+    {
+      // the last record ends where the domain does, at DBL_MAX
+      const double end = F.tend[r] < __builtin_inf() ? F.tend[r] : DBL_MAX;
+      const double len = end - (r > first ? F.tend[r - 1] : 0.0);
+      gv = 2.0 * ((v != 0.0 ? v * len * W : 0.0) - F.q[r]);
+      if (r > first)
+      {
+        const double vp = val(r - 1);
+        gt = (vp - v) * ((vp + v) * W - 2.0 * F.a[r - 1]);
+      }
+    }
+    const int64_t pt = F.src[r];
+    if (pt < 0) // the zero record standing in for an empty PCF
+      return;
+    if (F.grad_f64)
+    {
+      static_cast<double*>(F.grad_v)[pt] = gv;
+      static_cast<double*>(F.grad_t)[pt] = gt;
+    }
+    else
+    {
+      static_cast<float*>(F.grad_v)[pt] = static_cast<float>(gv);
+      static_cast<float*>(F.grad_t)[pt] = static_cast<float>(gt);
     }
   }
 
   size_t bwd_lds_bytes(int tv_f64, int R, int C, int cap)
   {
     (void)tv_f64;
-    return static_cast<size_t>(lds_layout(8, R, C, cap, 16).total);
+    return static_cast<size_t>(lds_layout(8, R, C, cap, 16).total) + 8 * static_cast<size_t>(R + C);
   }
 
   void launch_backward(const BwdParams& prm, int op, hipStream_t stream)
@@ -256,6 +298,15 @@ namespace sbamd
       else
         hipLaunchKernelGGL((bwd_tile_kernel<float, OPK_L2>), dim3(grid), block, lds, stream, prm);
     }
+    SBAMD_HIP(hipGetLastError());
+  }
+
+  void launch_finalize_grad(const FinalizeParams& prm, hipStream_t stream)
+  {
+    if (prm.records <= 0)
+      return;
+    const unsigned grid = static_cast<unsigned>((prm.records + 255) / 256);
+    hipLaunchKernelGGL(finalize_grad_kernel, dim3(grid), dim3(256), 0, stream, prm);
     SBAMD_HIP(hipGetLastError());
   }
 }

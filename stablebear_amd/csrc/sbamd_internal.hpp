@@ -89,15 +89,32 @@ namespace sbamd
 
   struct BwdParams
   {
-    FwdParams f;             // forward geometry (out unused)
-    const void* grad_out;    // upstream gradient per output element
-    const void* fwd_out;     // forward output (distances) for p = 2
-    const int64_t* r_src;    // record -> original flat point index
+    FwdParams f;          // forward geometry (out unused)
+    const void* grad_out; // upstream gradient per output element
+    const void* fwd_out;  // forward output (distances) for p = 2
+    double* q;            // per record: sum over pairs of w * integral of the partner over the record
+    double* a;            // per record: sum over pairs of w * partner value at the record's end
+    double* wsum;         // per PCF (sorted position): sum over its pairs of w
+    int32_t in_f64;       // grad_out / fwd_out are FP64
+    int32_t val_f64;      // record values are FP64 (else FP32); record times are FP64
+  };
+
+  struct FinalizeParams
+  {
+    const int64_t* roff; // sorted record offsets (n + 1)
+    const double* tend;  // record end times
+    const void* val;     // record values (FP64 or FP32)
+    const int64_t* src;  // record -> original point
+    const double* q;
+    const double* a;
+    const double* wsum;
     void* grad_v;
     void* grad_t;
+    int64_t n;
+    int64_t records;
+    int32_t val_f64;
     int32_t grad_f64;
-    int32_t in_f64;  // grad_out / fwd_out are FP64
-    int32_t val_f64; // record values are FP64 (else FP32); record times are FP64
+    int32_t op;
     int32_t pad;
   };
 
@@ -183,6 +200,9 @@ namespace sbamd
     // persistent launch: resident workgroups pull tiles from this counter
     bool persistent = false;
     DeviceBuffer counter;
+    // backward accumulators (per record q, a; per PCF wsum), allocated on first use
+    DeviceBuffer bwd_q, bwd_a, bwd_w;
+    int32_t bwd_cap = 32; // records staged per slot and phase in the backward
     uint64_t input_bytes = 0;
     void rebuild_tiles();
   };
@@ -192,6 +212,7 @@ namespace sbamd
   void launch_forward(const FwdParams& prm, int tt_f64, int tv_f64, int op, int kahan,
                       hipStream_t stream);
   void launch_backward(const BwdParams& prm, int op, hipStream_t stream);
+  void launch_finalize_grad(const FinalizeParams& prm, hipStream_t stream);
   size_t bwd_lds_bytes(int tv_f64, int R, int C, int cap);
   void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream);
   void launch_rectangles(const double* ft, const double* fv, int64_t mf, const double* gt,

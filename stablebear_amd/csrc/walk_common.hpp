@@ -292,8 +292,10 @@ namespace sbamd
 
   // First half of a phase (before the barrier): each slot proposes the phase
   // end (start time of its (cap+1)-th unconsumed record), and the workgroup
-  // copies up to cap records of every slot into LDS.
-  template <typename TT, typename TV>
+  // copies up to cap records of every slot into LDS. With LEAD = 1 the window
+  // also holds the record before the cursor (the record a walk is inside when
+  // the phase starts) at local index 0, and record cur + j at local j + 1.
+  template <typename TT, typename TV, int LEAD = 0>
   __device__ void phase_stage(const TileLds<TT, TV>& S, const FwdParams& P, int phase, int tid,
                               int nthreads)
   {
@@ -314,12 +316,13 @@ namespace sbamd
         tbs = (tid < P.R ? g_rt : g_ct)[S.base[tid] + c + cap - 1];
       atomicMin(&S.tb[phase & 1], L::bits(tbs));
     }
-    for (int idx = tid; idx < NS * cap; idx += nthreads)
+    constexpr int W = LEAD; // leading records before the cursor
+    for (int idx = tid; idx < NS * (cap + W); idx += nthreads)
     {
-      const int s = idx / cap;
-      const int j = idx - s * cap;
-      const int64_t k = S.next[s] + j;
-      if (k < S.m[s])
+      const int s = idx / (cap + W);
+      const int j = idx - s * (cap + W);
+      const int64_t k = S.next[s] - W + j;
+      if (k >= 0 && k < S.m[s])
       {
         const bool isrow = s < P.R;
         const int64_t gk = S.base[s] + k;
@@ -335,7 +338,7 @@ namespace sbamd
 
   // After the barrier: the slot threads find the first staged record that
   // starts at or after Tb (binary search) -- the next phase's cursor.
-  template <typename TT, typename TV>
+  template <typename TT, typename TV, int LEAD = 0>
   __device__ void phase_advance_cursor(const TileLds<TT, TV>& S, const FwdParams& P, TT Tb, int tid)
   {
     const int NS = P.R + P.C;
@@ -349,7 +352,7 @@ namespace sbamd
       while (lo < hi)
       {
         const int mid = (lo + hi) >> 1;
-        const TT ts = mid == 0 ? S.tstart[tid] : S.rec(lb + static_cast<uint32_t>(mid - 1) * ls).t;
+        const TT ts = mid == 0 ? S.tstart[tid] : S.rec(lb + static_cast<uint32_t>(mid - 1 + LEAD) * ls).t;
         if (ts < Tb)
           lo = mid + 1;
         else
@@ -357,7 +360,7 @@ namespace sbamd
       }
       S.next[tid] = c + lo;
       if (lo > 0)
-        S.tstart[tid] = S.rec(lb + static_cast<uint32_t>(lo - 1) * ls).t;
+        S.tstart[tid] = S.rec(lb + static_cast<uint32_t>(lo - 1 + LEAD) * ls).t;
     }
   }
 }
