@@ -1,15 +1,23 @@
 """Benchmark of the pairwise PCF integration hot path on one MI355X.
 
-Default workload (``--config c2``): L1 pdist of N = 20,000 PCFs with
-m ~ U{50..500} breakpoints (Exp(1) increments, N(0,1) values ending at 0),
-inputs resident in HBM. One step = one full pdist (all 199,990,000 pairs)
-in the FP32-compensated form (FP32C); the plain FP64 kernel is timed beside
-it, the maximum relative error of FP32C against the FP64 oracle is measured on
-a sample of rows, and the reference CPU walk is timed on a bounded sample of
-the same rows on the host cores.
+Default workload: L1 pdist of N = 20,000 PCFs with m ~ U{50..500}
+breakpoints (Exp(1) increments, N(0,1) values ending at 0), inputs resident
+in HBM. One step = one full pdist (all 199,990,000 pairs) in the
+FP32-compensated form (FP32C). Beside it, in the same run:
 
-Prints one JSON line (rank 0). ``--gpus`` above 1 is not supported: the run
-uses a single GPU.
+* the tiled FP64 form and the plain FP64 GPU kernel (one lane per pair over
+  raw breakpoints in global memory, the reference's kernel design) on the
+  same input, and the ratios of FP32C to both;
+* the maximum relative error of FP32C against the FP64 oracle on a sample of
+  rows, and bit-exactness of the tiled FP64 form on that sample;
+* the reference CPU walk (oracle/_ref, built from the reference's own
+  sources) timed on a bounded row sample on the host cores;
+* L2 pdist and the L2 kernel at the same N (FP32C and FP64, error on a
+  sample);
+* forward + backward of sum(W * D) for the L2 pdist at N = 4096.
+
+Prints one JSON line. ``--gpus`` above 1 is not supported: the run uses a
+single GPU.
 """
 
 from __future__ import annotations
@@ -26,9 +34,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"fp32c": 157.3, "fp64": 78.6}  # MI355X published vector peaks
+METRIC = "pdist pairs/sec (L1, fp32-compensated vs fp64) at ≤1e-6 rel err; fwd+bwd ms"
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}  # MI355X published peaks
 PEAK_HBM_GBS = 8000.0
 FLOPS_PER_SEGMENT = {"l1": 4, "l2": 5, "dot": 4}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_fwd_tile_l1_pdist_n20000.txt")
+
+
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def _cpu_model():
@@ -41,40 +55,49 @@ def _cpu_model():
     return platform.processor()
 
 
-def _time_plan(plan, out, f64, steps, warmup, stream):
+def _events_ms(fn, steps, warmup, stream):
+    """Average device ms of fn() over `steps` launches on `stream` (HIP events
+    recorded on that stream), after `warmup` untimed launches; the timed
+    region is bracketed by device synchronisations."""
     import torch
 
     for _ in range(warmup):
-        plan.execute(out.data_ptr(), f64, stream=stream.cuda_stream)
+        fn()
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record(stream)
     for _ in range(steps):
-        plan.execute(out.data_ptr(), f64, stream=stream.cuda_stream)
+        fn()
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return e0.elapsed_time(e1) / steps, wall
 
 
-def _sample_rows(n, target_pairs):
-    """First rows of the upper-row-major pdist loop (row i has n - i - 1 pairs)."""
+def _sample_rows(n, target_pairs, diag=False):
+    """Rows 0..r-1 of the row-major pair loop (row i has n - i - 1 pairs, or
+    n - i with the diagonal) covering at least target_pairs pairs."""
     acc = 0
     for r in range(n):
-        acc += n - r - 1
+        acc += n - r - (0 if diag else 1)
         if acc >= target_pairs:
             return r + 1
     return n
 
 
-def _storage_rows(n, r1):
-    """Compressed indices of pairs (i, j), i < r1, j > i (index j(j-1)/2 + i)."""
+def _storage_rows(n, r1, diag=False):
+    """Compressed indices of the pairs (i, j), i < r1, j > i (j >= i with the
+    diagonal): j(j-1)/2 + i (pdist) or j(j+1)/2 + i (kernel)."""
     idx = []
     for i in range(r1):
-        j = np.arange(i + 1, n, dtype=np.int64)
-        idx.append(j * (j - 1) // 2 + i)
+        if diag:
+            j = np.arange(i, n, dtype=np.int64)
+            idx.append(j * (j + 1) // 2 + i)
+        else:
+            j = np.arange(i + 1, n, dtype=np.int64)
+            idx.append(j * (j - 1) // 2 + i)
     return np.concatenate(idx) if idx else np.zeros(0, dtype=np.int64)
 
 
@@ -104,6 +127,79 @@ def cpu_baseline(off, t, v, n, cores, seconds_target=12.0):
                       f"{cores} threads, {_cpu_model()})"}, out, r1
 
 
+def _errors(got, exp):
+    nz = exp != 0
+    rel = float(np.max(np.abs(got[nz] - exp[nz]) / np.abs(exp[nz]))) if nz.any() else 0.0
+    scale = float(np.max(np.abs(exp))) if exp.size else 0.0
+    normwise = float(np.max(np.abs(got - exp)) / scale) if scale > 0 else 0.0
+    return rel, normwise
+
+
+def _traffic():
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
+    try:
+        vals = {}
+        for line in open(TRAFFIC_FILE):
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
+        return float(vals["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def _plain_fp64(csr, n, steps, stream):
+    """Plain FP64 GPU kernel (reference kernel design) on the same input."""
+    import torch
+
+    from stablebear_amd import _lib as L
+    from stablebear_amd import runtime
+
+    pk = runtime.PlainKernel(csr)
+    out = torch.empty(n * (n - 1) // 2, dtype=torch.float64, device="cuda")
+    ms, _ = _events_ms(lambda: pk.execute("l1", 1.0, L.OUT_DISTMAT, out.data_ptr(),
+                                          stream.cuda_stream), steps, 1, stream)
+    return ms, out
+
+
+def _other_ops(csr, off, t, v, n, steps, stream, cores, sample_pairs):
+    """L2 pdist and the L2 kernel at the same N: FP32C and FP64 timings and the
+    errors of both against the FP64 oracle on the first rows."""
+    import torch
+
+    import oracle
+    from stablebear_amd import runtime
+
+    res = {}
+    for name, op, p, kind in (("l2_pdist", "lp", 2.0, oracle.KIND_PDIST),
+                              ("l2_kernel", "dot", 1.0, oracle.KIND_KERNEL)):
+        diag = kind == oracle.KIND_KERNEL
+        r1 = _sample_rows(n, sample_pairs, diag)
+        exp = oracle.pairwise(oracle.OP_LP if op == "lp" else oracle.OP_DOT, p, kind, off, t, v,
+                              nthreads=cores, rows=(0, r1))
+        idx = _storage_rows(n, r1, diag)
+        exp = exp[idx]
+        entry = {"sample_rows": r1}
+        for prec in ("fp32c", "fp64"):
+            ds = runtime.DeviceSet(csr, prec)
+            plan = runtime.Plan(ds, op=op, p=p)  # "dot" -> symmetric matrix with diagonal
+            f64 = prec == "fp64"
+            out = torch.empty(plan.out_count, dtype=torch.float64 if f64 else torch.float32,
+                              device="cuda")
+            ms, _ = _events_ms(lambda: plan.execute(out.data_ptr(), f64,
+                                                    stream=stream.cuda_stream),
+                               steps, 1, stream)
+            got = out[torch.from_numpy(idx).cuda()].double().cpu().numpy()
+            rel, normwise = _errors(got, exp)
+            entry[prec] = {"ms": ms, "pairs_per_s": plan.pairs / (ms * 1e-3),
+                           "max_rel_err": rel, "normwise_err": normwise}
+            del out, plan, ds
+        entry["fp32c_over_fp64"] = entry["fp64"]["ms"] / entry["fp32c"]["ms"]
+        res[name] = entry
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,8 +207,15 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-fwdbwd", action="store_true", help="skip the C4 fwd+bwd leg")
+    ap.add_argument("--no-fwdbwd", action="store_true", help="skip the fwd+bwd leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 kernel leg")
+    ap.add_argument("--no-ops", action="store_true", help="skip the L2 / kernel legs")
     ap.add_argument("--autotune", action="store_true", help="autotune tile shapes first")
+    ap.add_argument("--only-fwdbwd", action="store_true",
+                    help="run only the fwd+bwd leg (for profiling the backward)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device-to-device copy (the reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -124,10 +227,22 @@ def main(argv=None):
 
     from stablebear_amd import runtime, workloads
 
+    stream = torch.cuda.current_stream()
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 4}), flush=True)
+        return 0
+    if args.only_fwdbwd:
+        print(json.dumps({"fwd_bwd": _fwd_bwd_c4(stream)}), flush=True)
+        return 0
+
     n = args.n
     off, t, v = workloads.c2(n)
     csr = runtime.csr_from_arrays(off, t, v)
-    stream = torch.cuda.current_stream()
 
     results = {}
     for prec in ("fp32c", "fp64"):
@@ -138,8 +253,10 @@ def main(argv=None):
                           device="cuda")
         if args.autotune:
             plan.autotune(out.data_ptr(), f64, reps=1)
-        ms, wall = _time_plan(plan, out, f64, args.steps, args.warmup, stream)
+        ms, wall = _events_ms(lambda: plan.execute(out.data_ptr(), f64, stream=stream.cuda_stream),
+                              args.steps, args.warmup, stream)
         results[prec] = {"ms": ms, "wall_s": wall, "plan": plan, "out": out, "set": ds}
+        _progress(f"L1 pdist {prec}: {ms:.2f} ms/step")
 
     fp = results["fp32c"]
     plan = fp["plan"]
@@ -148,36 +265,59 @@ def main(argv=None):
     value = pairs / (ms * 1e-3)
     fp64_value = pairs / (results["fp64"]["ms"] * 1e-3)
 
-    # roofline of the dominant kernel (algorithmic FLOPs and bytes per launch)
+    plain_ms = None
+    plain_out = None
+    if not args.no_plain:
+        plain_ms, plain_out = _plain_fp64(csr, n, max(1, min(args.steps, 2)), stream)
+        _progress(f"plain FP64 kernel: {plain_ms:.2f} ms")
+
+    # roofline of the dominant kernel (one launch = one whole pdist)
     flops = FLOPS_PER_SEGMENT["l1"] * plan.segments
     achieved = flops / (ms * 1e-3) / 1e12
     in_bytes = int(off[-1]) * (8 + 4) + 8 * (n + 1)
     alg_bytes = in_bytes + pairs * 4
-    roofline = {"bound": "mfma", "compute_unit": "valu", "achieved": achieved,
-                "peak": PEAK_TFLOPS["fp32c"], "unit": "TFLOP/s",
-                "frac": achieved / PEAK_TFLOPS["fp32c"], "traffic": None,
+    roofline = {"bound": "mfma", "compute_unit": "valu (no MFMA: the walk is compare/select/"
+                "FMA work on the vector ALUs; peak = MI355X FP32 vector rate)",
+                "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+                "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": _traffic(),
                 "flops_per_segment": FLOPS_PER_SEGMENT["l1"], "segments": plan.segments,
+                "alg_bytes": alg_bytes,
                 "hbm_alg_GBs": alg_bytes / (ms * 1e-3) / 1e9,
                 "hbm_alg_frac": alg_bytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
 
     cores = len(os.sched_getaffinity(0))
     cpu = None
     max_rel = None
+    fp64_bitexact = None
+    plain_bitexact = None
     if not args.no_cpu:
         cpu, ref_rows, r1 = cpu_baseline(off, t, v, n, cores)
+        _progress(f"CPU baseline: {cpu['value']:.3g} pairs/s on {cores} threads")
         idx = _storage_rows(n, r1)
         exp = ref_rows[idx]
-        got32 = fp["out"][torch.from_numpy(idx).cuda()].double().cpu().numpy()
-        got64 = results["fp64"]["out"][torch.from_numpy(idx).cuda()].cpu().numpy()
-        nz = exp != 0
-        max_rel = float(np.max(np.abs(got32[nz] - exp[nz]) / np.abs(exp[nz])))
+        idx_d = torch.from_numpy(idx).cuda()
+        got32 = fp["out"][idx_d].double().cpu().numpy()
+        got64 = results["fp64"]["out"][idx_d].cpu().numpy()
+        max_rel, _ = _errors(got32, exp)
         fp64_bitexact = bool(np.array_equal(got64, exp))
+        if plain_out is not None:
+            plain_bitexact = bool(np.array_equal(plain_out[idx_d].cpu().numpy(), exp))
+    del plain_out
+    for r in results.values():
+        del r["out"]
+    torch.cuda.empty_cache()
+
+    ops = None
+    if not args.no_ops:
+        ops = _other_ops(csr, off, t, v, n, max(1, min(args.steps, 3)), stream, cores, 300_000)
+        _progress("L2 pdist / L2 kernel done")
     fwdbwd = None
     if not args.no_fwdbwd:
         fwdbwd = _fwd_bwd_c4(stream)
+        _progress(f"fwd+bwd: ratio {fwdbwd['ratio']:.2f}")
 
     line = {
-        "metric": "pdist pairs/sec (L1, fp32-compensated vs fp64) at <=1e-6 rel err; fwd+bwd ms",
+        "metric": METRIC,
         "value": value,
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -187,20 +327,27 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32c (f64 compare/dt, f32 products, Kahan f32 sum)",
-        "data": "synthetic (seeded C2 recipe)",
-        "config": {"workload": f"L1 pdist, N={n}, m~U{{50..500}}, Exp(1) increments, "
-                               "N(0,1) values ending at 0",
+        "dtype": "fp32c (f64 compare/dt, f32 products, Kahan f32 sum, f64 final sum)",
+        "data": "synthetic (seeded: m~U{50..500}, Exp(1) increments, N(0,1) values ending at 0)",
+        "config": {"workload": f"L1 pdist, N={n}, m~U{{50..500}}",
                    "pairs": pairs, "segments": plan.segments, "tile": list(plan.shape)},
-        "fp64_pairs_per_s": fp64_value,
-        "fp64_ms": results["fp64"]["ms"],
-        "fp32c_over_fp64": value / fp64_value,
+        "fp64_tiled_pairs_per_s": fp64_value,
+        "fp64_tiled_ms": results["fp64"]["ms"],
+        "fp32c_over_fp64_tiled": value / fp64_value,
         "max_rel_err_vs_fp64_oracle": max_rel,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if plain_ms is not None:
+        line["plain_fp64_pairs_per_s"] = pairs / (plain_ms * 1e-3)
+        line["plain_fp64_ms"] = plain_ms
+        line["fp32c_over_plain_fp64"] = plain_ms / ms
     if not args.no_cpu:
-        line["fp64_bitexact_on_sample"] = fp64_bitexact
+        line["fp64_tiled_bitexact_on_sample"] = fp64_bitexact
+        if plain_bitexact is not None:
+            line["plain_fp64_bitexact_on_sample"] = plain_bitexact
+    if ops is not None:
+        line["ops"] = ops
     if fwdbwd is not None:
         line["fwd_bwd"] = fwdbwd
     print(json.dumps(line), flush=True)
@@ -208,7 +355,8 @@ def main(argv=None):
 
 
 def _fwd_bwd_c4(stream):
-    """C4: L2 pdist forward + backward of sum(W * D), N = 4096, FP32C forward."""
+    """L2 pdist forward + backward of sum(W * D), N = 4096, m ~ U{20..200},
+    FP32C forward and FP64-accumulated backward."""
     import torch
 
     from stablebear_amd import runtime, workloads

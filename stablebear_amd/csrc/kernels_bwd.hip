@@ -147,11 +147,11 @@ namespace sbamd
             const double rw = w * static_cast<double>(r.v);
             const uint32_t np = lo_p + REC;
             const bool k = r.t <= hi_t;
-            const TT nlt = k ? r.t : hi_t;
+            const TT nlt = tmin(r.t, hi_t);
             const double nlw = k ? rw : hi_w;
             const uint32_t nlp = k ? np : hi_p;
             const double nla = k ? 0.0 : hi_acc;
-            hi_t = k ? hi_t : r.t;
+            hi_t = tmax(r.t, hi_t);
             hi_w = k ? hi_w : rw;
             hi_p = k ? hi_p : np;
             hi_acc = k ? hi_acc : 0.0;

@@ -18,8 +18,9 @@ namespace sbamd
 {
   template <typename TT, typename TV, int OP, bool KAHAN>
   // 8 waves per SIMD (at most 64 VGPRs): the walk is latency-bound on its
-  // dependent LDS loads, so residency matters more than registers.
-  __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void fwd_tile_kernel(FwdParams P)
+  // dependent LDS loads, so residency matters more than registers. The
+  // general-p operation carries an inlined pow, which would spill at 64 VGPRs.
+  __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OP == OPK_LP ? 4 : 8, 8))) void fwd_tile_kernel(FwdParams P)
   {
     extern __shared__ __align__(16) unsigned char smem[];
     using L = Lim<TT>;
@@ -69,10 +70,15 @@ namespace sbamd
       // union breakpoint is always lo.t, and the side that owns lo advances.
       // (Which PCF lo belongs to does not matter to a symmetric pair op.)
       // lo_p / hi_p: LDS address of the owning PCF's next record; between
-      // phases they are kept as (slot, record index) instead.
-      TT t = TT(0);
+      // phases they are kept as (slot, record index) instead. x is the first
+      // half of the pair op of the current segment (pair_pre); lo's value is
+      // not kept, since after a step the two current values are {new
+      // record's value, hi_v} whichever side advanced, and every pair op is
+      // symmetric in its operands. len is the length of the current segment
+      // [previous breakpoint, lo_t), formed in the time type when the
+      // segment starts and rounded to the value type.
       TT lo_t = L::inf(), hi_t = L::inf();
-      TV lo_v = TV(0), hi_v = TV(0);
+      TV len = TV(0), x = TV(0), hi_v = TV(0);
       int lo_s = slot_f, hi_s = slot_g;  // owning slots
       int64_t lo_k = 1, hi_k = 1;        // record index of each owner's next record
       using Acc = typename std::conditional<KAHAN, KahanAcc, PlainAcc<TV>>::type;
@@ -93,9 +99,10 @@ namespace sbamd
             const RecT rg = S.rec(S.lbase[slot_g]);
             const bool k = rf.t <= rg.t;
             lo_t = k ? rf.t : rg.t;
-            lo_v = k ? rf.v : rg.v;
             hi_t = k ? rg.t : rf.t;
             hi_v = k ? rg.v : rf.v;
+            x = pair_pre<OP, TV>(rf.v, rg.v);
+            len = static_cast<TV>(lo_t - TT(0));
             lo_s = k ? slot_f : slot_g;
             hi_s = k ? slot_g : slot_f;
           }
@@ -103,24 +110,23 @@ namespace sbamd
           const RecT* const recs = reinterpret_cast<const RecT*>(smem);
           const RecT* lo_p = recs + (S.lbase[lo_s] / SF + static_cast<uint32_t>(lo_k - lo_c));
           const RecT* hi_p = recs + (S.lbase[hi_s] / SF + static_cast<uint32_t>(hi_k - hi_c));
-          // One merge step: accumulate [t, lo.t), then replace lo by the next
-          // record of its PCF and restore the order. Two steps per trip let
-          // the registers of t, lo and the accumulator alternate roles.
+          // One merge step: accumulate the segment ending at lo.t, then
+          // replace lo by the next record of its PCF and restore the order.
+          // Two steps per trip let registers alternate roles.
           auto step = [&]() {
-            acc.add(static_cast<TV>(lo_t - t), pair_op<OP, TV>(lo_v, hi_v, p));
-            t = lo_t;
+            acc.add(len, pair_post<OP, TV>(x, p));
             const RecT r = *lo_p;
             const RecT* np = lo_p + 1;
             const bool k = r.t <= hi_t;
-            const TT nlt = k ? r.t : hi_t;
-            const TV nlv = k ? r.v : hi_v;
-            const RecT* nlp = k ? np : hi_p;
-            hi_t = k ? hi_t : r.t;
-            hi_v = k ? hi_v : r.v;
-            hi_p = k ? hi_p : np;
+            x = pair_pre<OP, TV>(r.v, hi_v);
+            const TT nlt = tmin(r.t, hi_t);
+            len = static_cast<TV>(nlt - lo_t);
             lo_t = nlt;
-            lo_v = nlv;
+            hi_t = tmax(r.t, hi_t);
+            const RecT* nlp = k ? np : hi_p;
+            hi_p = k ? hi_p : np;
             lo_p = nlp;
+            hi_v = k ? hi_v : r.v;
           };
           while (lo_t < Tb)
           {
@@ -147,9 +153,15 @@ namespace sbamd
 
       if (valid)
       {
-        const TV w = pair_op<OP, TV>(lo_v, hi_v, p);
+        // final rectangle [t, MAX) from the last union breakpoint t: the
+        // later of the two PCFs' last record starts
+        auto last_start = [&](int s, const void* tend) {
+          const int64_t m = S.m[s];
+          return m >= 2 ? static_cast<const TT*>(tend)[S.base[s] + m - 2] : TT(0);
+        };
+        const TT t = tmax(last_start(slot_f, P.r_tend), last_start(slot_g, P.c_tend));
         const TT tail = L::max() - t;
-        const double raw = acc.finish(tail, w);
+        const double raw = acc.finish(tail, pair_post<OP, TV>(x, p));
         const double res = finalize<OP>(raw, P.p);
         const int64_t i = P.r_perm[pr.row_base + tl.r0 + lr];
         const int64_t j = P.c_perm[pr.col_base + tl.c0 + lc];

@@ -47,6 +47,35 @@ namespace sbamd
     __device__ static float from_bits(Bits b) { return __uint_as_float(b); }
   };
 
+  // Minimum and maximum of two breakpoint times, one v_min / v_max each.
+  // Times are never NaN (inputs are validated finite; +inf is the end
+  // sentinel), so the operand canonicalisation the compiler wraps around
+  // fmin / fmax for IEEE-mode NaN quieting is dead weight in the walk.
+  __device__ __forceinline__ double tmin(double a, double b)
+  {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+  }
+  __device__ __forceinline__ double tmax(double a, double b)
+  {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+  }
+  __device__ __forceinline__ float tmin(float a, float b)
+  {
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+  }
+  __device__ __forceinline__ float tmax(float a, float b)
+  {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+  }
+
   // Pair operations: operations.cuh:19-65. OPK_L2 is the Lp operation at
   // p == 2 (|d|^2 computed as d * d, finalize sqrt).
   template <int OP, typename TV>
@@ -63,6 +92,34 @@ namespace sbamd
       return pow(fabs(f - g), p);
     else
       return f * g;
+  }
+
+  // The same operations split in two, for a walk that keeps the pair's
+  // current segment as loop state: pair_pre(f, g) is the part formed when the
+  // segment starts (f - g, or f * g), pair_post(x, p) the rest, applied where
+  // the segment is accumulated, so that |x| folds into the consuming FMA as an
+  // operand modifier. pair_post(pair_pre(f, g), p) == pair_op(f, g, p) bit for
+  // bit, and pair_pre(f, g) == +-pair_pre(g, f), which pair_post erases.
+  template <int OP, typename TV>
+  __device__ __forceinline__ TV pair_pre(TV f, TV g)
+  {
+    if constexpr (OP == OPK_DOT)
+      return f * g;
+    else
+      return f - g;
+  }
+
+  template <int OP, typename TV>
+  __device__ __forceinline__ TV pair_post(TV x, TV p)
+  {
+    if constexpr (OP == OPK_L1)
+      return fabs(x);
+    else if constexpr (OP == OPK_L2)
+      return x * x;
+    else if constexpr (OP == OPK_LP)
+      return pow(fabs(x), p);
+    else
+      return x;
   }
 
   template <int OP>
