@@ -166,6 +166,37 @@ namespace sbamd
         fail(SBAMD_ERR_INVALID, "unknown precision");
     }
 
+    // Device record array: {t_end, value} per record in the set's types; the
+    // value sits at byte offset record_bytes / 2 (FP32 values pad to 16 bytes
+    // behind FP64 times, so one record is one aligned 16-byte load).
+    std::vector<unsigned char> interleave_records(const PackedSet& pk, int32_t tt_f64, int32_t tv_f64)
+    {
+      if (!tt_f64 && tv_f64)
+        fail(SBAMD_ERR_RUNTIME, "FP64 values need FP64 times");
+      const size_t R = pk.tend64.size();
+      const size_t rb = static_cast<size_t>(record_bytes(tt_f64));
+      std::vector<unsigned char> out(R * rb, 0);
+      for (size_t r = 0; r < R; ++r)
+      {
+        unsigned char* p = out.data() + r * rb;
+        if (tt_f64)
+          std::memcpy(p, &pk.tend64[r], 8);
+        else
+        {
+          const float t = static_cast<float>(pk.tend64[r]);
+          std::memcpy(p, &t, 4);
+        }
+        if (tv_f64)
+          std::memcpy(p + rb / 2, &pk.val64[r], 8);
+        else
+        {
+          const float v = static_cast<float>(pk.val64[r]);
+          std::memcpy(p + rb / 2, &v, 4);
+        }
+      }
+      return out;
+    }
+
     void make_set(Ctx* ctx, const sbamd_pcf_set& in, int32_t precision, Set& s)
     {
       PackedSet pk;
@@ -184,20 +215,7 @@ namespace sbamd
       upload(s.roff, pk.roff);
       upload(s.perm, pk.perm);
       upload(s.src, pk.src);
-      if (s.tt_f64)
-        upload(s.tend, pk.tend64);
-      else
-      {
-        std::vector<float> tf(pk.tend64.begin(), pk.tend64.end());
-        upload(s.tend, tf);
-      }
-      if (s.tv_f64)
-        upload(s.val, pk.val64);
-      else
-      {
-        std::vector<float> vf(pk.val64.begin(), pk.val64.end());
-        upload(s.val, vf);
-      }
+      upload(s.rec, interleave_records(pk, s.tt_f64, s.tv_f64));
     }
 
     std::vector<int64_t> sorted_counts(const Set& s)
@@ -228,23 +246,19 @@ namespace sbamd
       if (pl.batched)
       {
         P.r_off = pl.b_roff.as<int64_t>();
-        P.r_tend = pl.b_tend.ptr;
-        P.r_val = pl.b_val.ptr;
+        P.r_rec = pl.b_rec.ptr;
         P.r_perm = pl.b_perm.as<int64_t>();
         P.c_off = P.r_off;
-        P.c_tend = P.r_tend;
-        P.c_val = P.r_val;
+        P.c_rec = P.r_rec;
         P.c_perm = P.r_perm;
       }
       else
       {
         P.r_off = rs->roff.as<int64_t>();
-        P.r_tend = rs->tend.ptr;
-        P.r_val = rs->val.ptr;
+        P.r_rec = rs->rec.ptr;
         P.r_perm = rs->perm.as<int64_t>();
         P.c_off = cs->roff.as<int64_t>();
-        P.c_tend = cs->tend.ptr;
-        P.c_val = cs->val.ptr;
+        P.c_rec = cs->rec.ptr;
         P.c_perm = cs->perm.as<int64_t>();
       }
       P.tiles = pl.tiles.as<Tile>();
@@ -501,8 +515,9 @@ extern "C"
       for (int64_t r : set->h_src)
         if (r < 0)
           fail(SBAMD_ERR_INVALID, "value updates need sets without empty PCFs");
-      launch_update_values(v_device, set->src.as<int64_t>(), set->records, set->val.ptr,
-                           set->in_dtype == SBAMD_F64, set->tv_f64, pick_stream(set->ctx, stream));
+      launch_update_values(v_device, set->src.as<int64_t>(), set->records, set->rec.ptr,
+                           record_bytes(set->tt_f64), set->in_dtype == SBAMD_F64, set->tv_f64,
+                           pick_stream(set->ctx, stream));
     });
   }
 
@@ -562,7 +577,7 @@ extern "C"
       pl->h_probs.push_back(pr);
       pl->input_bytes = 0;
       for (const Set* s : pl->sets)
-        pl->input_bytes += s->tend.bytes + s->val.bytes + s->roff.bytes;
+        pl->input_bytes += s->rec.bytes + s->roff.bytes;
       if (shape)
       {
         if (shape->rows)
@@ -639,24 +654,21 @@ extern "C"
       }
       pl->out_count = out_base;
       use_device(ctx);
-      const size_t tb = pl->tt_f64 ? 8 : 4, vb = pl->tv_f64 ? 8 : 4;
-      pl->b_tend = DeviceBuffer(static_cast<size_t>(rec_base) * tb);
-      pl->b_val = DeviceBuffer(static_cast<size_t>(rec_base) * vb);
+      const size_t rb = static_cast<size_t>(record_bytes(pl->tt_f64));
+      pl->b_rec = DeviceBuffer(static_cast<size_t>(rec_base) * rb);
       int64_t at = 0;
       for (int32_t b = 0; b < k; ++b)
       {
         const Set* s = sets[b];
         if (s->records == 0)
           continue;
-        SBAMD_HIP(hipMemcpy(static_cast<char*>(pl->b_tend.ptr) + at * tb, s->tend.ptr,
-                            static_cast<size_t>(s->records) * tb, hipMemcpyDeviceToDevice));
-        SBAMD_HIP(hipMemcpy(static_cast<char*>(pl->b_val.ptr) + at * vb, s->val.ptr,
-                            static_cast<size_t>(s->records) * vb, hipMemcpyDeviceToDevice));
+        SBAMD_HIP(hipMemcpy(static_cast<char*>(pl->b_rec.ptr) + at * rb, s->rec.ptr,
+                            static_cast<size_t>(s->records) * rb, hipMemcpyDeviceToDevice));
         at += s->records;
       }
       upload(pl->b_roff, roff_all);
       upload(pl->b_perm, perm_all);
-      pl->input_bytes = pl->b_tend.bytes + pl->b_val.bytes + pl->b_roff.bytes;
+      pl->input_bytes = pl->b_rec.bytes + pl->b_roff.bytes;
       if (shape)
       {
         if (shape->rows)
@@ -897,8 +909,7 @@ extern "C"
       FinalizeParams F;
       std::memset(&F, 0, sizeof(F));
       F.roff = s->roff.as<int64_t>();
-      F.tend = s->tend.as<double>();
-      F.val = s->val.ptr;
+      F.rec = s->rec.ptr;
       F.src = s->src.as<int64_t>();
       F.q = B.q;
       F.a = B.a;

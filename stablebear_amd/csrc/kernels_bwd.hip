@@ -230,9 +230,12 @@ namespace sbamd
     }
     const int64_t q = lo;
     const int64_t first = F.roff[q];
+    // 16-byte records: FP64 end time, then the value (FP64 or FP32)
+    const unsigned char* rec = static_cast<const unsigned char*>(F.rec);
+    auto tend = [&](int64_t k) { return *reinterpret_cast<const double*>(rec + 16 * k); };
     auto val = [&](int64_t k) {
-      return F.val_f64 ? static_cast<const double*>(F.val)[k]
-                       : static_cast<double>(static_cast<const float*>(F.val)[k]);
+      return F.val_f64 ? *reinterpret_cast<const double*>(rec + 16 * k + 8)
+                       : static_cast<double>(*reinterpret_cast<const float*>(rec + 16 * k + 8));
     };
     const double v = val(r);
     const double W = F.wsum[q];
@@ -246,8 +249,8 @@ namespace sbamd
     else
     {
       // the last record ends where the domain does, at DBL_MAX
-      const double end = F.tend[r] < __builtin_inf() ? F.tend[r] : DBL_MAX;
-      const double len = end - (r > first ? F.tend[r - 1] : 0.0);
+      const double end = tend(r) < __builtin_inf() ? tend(r) : DBL_MAX;
+      const double len = end - (r > first ? tend(r - 1) : 0.0);
       gv = 2.0 * ((v != 0.0 ? v * len * W : 0.0) - F.q[r]);
       if (r > first)
       {

@@ -135,12 +135,15 @@ namespace sbamd
               break;
             step();
           }
-          // back to (slot, record index): slots are slot_recs records apart
-          const uint32_t SR = static_cast<uint32_t>(slot_recs(P.cap));
+          // back to (slot, record index). A next-record pointer lies within
+          // local 0 .. cap of its slot's window (slot_recs > cap), and the
+          // row slot precedes the column slot, so one compare with the
+          // column window's start tells the owners apart.
           const uint32_t lo_i = static_cast<uint32_t>(lo_p - recs);
           const uint32_t hi_i = static_cast<uint32_t>(hi_p - recs);
-          lo_s = static_cast<int>(lo_i / SR);
-          hi_s = static_cast<int>(hi_i / SR);
+          const bool lo_in_g = lo_i >= S.lbase[slot_g] / SF;
+          lo_s = lo_in_g ? slot_g : slot_f;
+          hi_s = lo_in_g ? slot_f : slot_g;
           lo_k = S.cur[lo_s] + (lo_i - S.lbase[lo_s] / SF);
           hi_k = S.cur[hi_s] + (hi_i - S.lbase[hi_s] / SF);
           if (fold)
@@ -155,11 +158,11 @@ namespace sbamd
       {
         // final rectangle [t, MAX) from the last union breakpoint t: the
         // later of the two PCFs' last record starts
-        auto last_start = [&](int s, const void* tend) {
+        auto last_start = [&](int s, const void* rec) {
           const int64_t m = S.m[s];
-          return m >= 2 ? static_cast<const TT*>(tend)[S.base[s] + m - 2] : TT(0);
+          return m >= 2 ? static_cast<const RecT*>(rec)[S.base[s] + m - 2].t : TT(0);
         };
-        const TT t = tmax(last_start(slot_f, P.r_tend), last_start(slot_g, P.c_tend));
+        const TT t = tmax(last_start(slot_f, P.r_rec), last_start(slot_g, P.c_rec));
         const TT tail = L::max() - t;
         const double raw = acc.finish(tail, pair_post<OP, TV>(x, p));
         const double res = finalize<OP>(raw, P.p);

@@ -99,33 +99,36 @@ namespace sbamd
     SBAMD_HIP(hipGetLastError());
   }
 
+  // rec: record array of rec_bytes-byte records, value at byte rec_bytes / 2.
   template <typename TI, typename TO>
   __global__ void gather_values_kernel(const TI* __restrict__ v, const int64_t* __restrict__ src,
-                                       int64_t records, TO* __restrict__ rec)
+                                       int64_t records, unsigned char* __restrict__ rec,
+                                       int rec_bytes)
   {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < records)
-      rec[i] = static_cast<TO>(v[src[i]]);
+      *reinterpret_cast<TO*>(rec + i * rec_bytes + rec_bytes / 2) = static_cast<TO>(v[src[i]]);
   }
 
-  void launch_update_values(const void* v, const int64_t* src, int64_t records, void* rec_val,
-                            int in_f64, int out_f64, hipStream_t stream)
+  void launch_update_values(const void* v, const int64_t* src, int64_t records, void* rec,
+                            int rec_bytes, int in_f64, int out_f64, hipStream_t stream)
   {
     if (records <= 0)
       return;
     const unsigned grid = static_cast<unsigned>((records + 255) / 256);
+    unsigned char* r = static_cast<unsigned char*>(rec);
     if (in_f64 && out_f64)
       hipLaunchKernelGGL((gather_values_kernel<double, double>), dim3(grid), dim3(256), 0, stream,
-                         static_cast<const double*>(v), src, records, static_cast<double*>(rec_val));
+                         static_cast<const double*>(v), src, records, r, rec_bytes);
     else if (in_f64)
       hipLaunchKernelGGL((gather_values_kernel<double, float>), dim3(grid), dim3(256), 0, stream,
-                         static_cast<const double*>(v), src, records, static_cast<float*>(rec_val));
+                         static_cast<const double*>(v), src, records, r, rec_bytes);
     else if (out_f64)
       hipLaunchKernelGGL((gather_values_kernel<float, double>), dim3(grid), dim3(256), 0, stream,
-                         static_cast<const float*>(v), src, records, static_cast<double*>(rec_val));
+                         static_cast<const float*>(v), src, records, r, rec_bytes);
     else
       hipLaunchKernelGGL((gather_values_kernel<float, float>), dim3(grid), dim3(256), 0, stream,
-                         static_cast<const float*>(v), src, records, static_cast<float*>(rec_val));
+                         static_cast<const float*>(v), src, records, r, rec_bytes);
     SBAMD_HIP(hipGetLastError());
   }
 

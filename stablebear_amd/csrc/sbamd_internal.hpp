@@ -65,12 +65,10 @@ namespace sbamd
   struct FwdParams
   {
     const int64_t* r_off;  // record offsets of the row set (sorted PCF order), n+1 per problem base
-    const void* r_tend;    // record end times (TT)
-    const void* r_val;     // record values (TV)
+    const void* r_rec;     // records {t_end (TT), value (TV)}, 16 bytes (FP64 times) or 8
     const int64_t* r_perm; // sorted position -> original index within its problem
     const int64_t* c_off;
-    const void* c_tend;
-    const void* c_val;
+    const void* c_rec;
     const int64_t* c_perm;
     const Tile* tiles;
     const Problem* probs;
@@ -102,8 +100,7 @@ namespace sbamd
   struct FinalizeParams
   {
     const int64_t* roff; // sorted record offsets (n + 1)
-    const double* tend;  // record end times
-    const void* val;     // record values (FP64 or FP32)
+    const void* rec;     // records {t_end (FP64), value (FP64 or FP32)}, 16 bytes each
     const int64_t* src;  // record -> original point
     const double* q;
     const double* a;
@@ -174,7 +171,9 @@ namespace sbamd
     std::vector<int64_t> h_roff; // sorted record offsets
     std::vector<int64_t> h_perm;
     std::vector<int64_t> h_src;
-    DeviceBuffer roff, tend, val, perm, src;
+    // rec: one record per constant piece, {t_end, value} in the set's time and
+    // value types (16 bytes with FP64 times, 8 with FP32 times), sorted order
+    DeviceBuffer roff, rec, perm, src;
     DeviceBuffer inv_src; // original point -> record (or -1), for value updates
   };
 
@@ -196,7 +195,7 @@ namespace sbamd
     std::vector<Tile> h_tiles;
     DeviceBuffer probs, tiles;
     // batched: concatenated set arrays
-    DeviceBuffer b_roff, b_tend, b_val, b_perm;
+    DeviceBuffer b_roff, b_rec, b_perm;
     // persistent launch: resident workgroups pull tiles from this counter
     bool persistent = false;
     DeviceBuffer counter;
@@ -221,8 +220,13 @@ namespace sbamd
   void launch_rectangles_f32(const float* ft, const float* fv, int64_t mf, const float* gt,
                              const float* gv, int64_t mg, float a, float b, float* out,
                              int64_t cap, int64_t* count, hipStream_t stream);
-  void launch_update_values(const void* v, const int64_t* src, int64_t records, void* rec_val,
-                            int in_f64, int out_f64, hipStream_t stream);
+  // Writes new values into the value field of a record array (records of
+  // rec_bytes bytes, the value at byte offset rec_bytes / 2).
+  void launch_update_values(const void* v, const int64_t* src, int64_t records, void* rec,
+                            int rec_bytes, int in_f64, int out_f64, hipStream_t stream);
+
+  // Bytes of one device record for the given time type.
+  inline int record_bytes(int tt_f64) { return tt_f64 ? 16 : 8; }
   void launch_plain(const int64_t* off, const double* t, const double* v, int64_t n, int op,
                     double p, int out_kind, double* out, hipStream_t stream);
 

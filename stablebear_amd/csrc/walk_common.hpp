@@ -347,22 +347,52 @@ namespace sbamd
     }
   }
 
+  // One 16-byte global -> LDS load (global_load_lds_dwordx4). lds_base is the
+  // wavefront-uniform LDS address of lane 0; lane i's record lands at
+  // lds_base + 16 i (lanes that do not call leave their slot untouched).
+  // The builtin exists only in the device compilation.
+  __device__ __forceinline__ void lds_dma16(const void* src, unsigned char* lds_base)
+  {
+#if __HIP_DEVICE_COMPILE__
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)lds_base, 16, 0, 0);
+#else
+    (void)src;
+    (void)lds_base;
+#endif
+  }
+
+  // A value every lane of the wavefront holds (read from one LDS address),
+  // moved to a scalar register so that the address arithmetic on it is
+  // scalar.
+  __device__ __forceinline__ int64_t wave_uniform(int64_t x)
+  {
+    const uint64_t u = static_cast<uint64_t>(x);
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(u)));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(u >> 32)));
+    return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+  }
+  __device__ __forceinline__ uint32_t wave_uniform(uint32_t x)
+  {
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
+  }
+
   // First half of a phase (before the barrier): each slot proposes the phase
   // end (start time of its (cap+1)-th unconsumed record), and the workgroup
-  // copies up to cap records of every slot into LDS. With LEAD = 1 the window
-  // also holds the record before the cursor (the record a walk is inside when
-  // the phase starts) at local index 0, and record cur + j at local j + 1.
+  // copies up to cap records of every slot into LDS, one wavefront per slot
+  // (a slot's window is contiguous in global memory and in LDS, so each lane
+  // moves whole 16- or 8-byte records). With LEAD = 1 the window also holds
+  // the record before the cursor (the record a walk is inside when the phase
+  // starts) at local index 0, and record cur + j at local j + 1.
   template <typename TT, typename TV, int LEAD = 0>
   __device__ void phase_stage(const TileLds<TT, TV>& S, const FwdParams& P, int phase, int tid,
                               int nthreads)
   {
     using L = Lim<TT>;
+    using RecT = Rec<TT, TV>;
     const int NS = P.R + P.C;
     const int cap = P.cap;
-    const TT* __restrict__ g_rt = static_cast<const TT*>(P.r_tend);
-    const TV* __restrict__ g_rv = static_cast<const TV*>(P.r_val);
-    const TT* __restrict__ g_ct = static_cast<const TT*>(P.c_tend);
-    const TV* __restrict__ g_cv = static_cast<const TV*>(P.c_val);
+    const RecT* __restrict__ g_r = static_cast<const RecT*>(P.r_rec);
+    const RecT* __restrict__ g_c = static_cast<const RecT*>(P.c_rec);
     if (tid < NS)
     {
       const int64_t c = S.next[tid];
@@ -370,25 +400,42 @@ namespace sbamd
       const int64_t rem = S.m[tid] - c;
       TT tbs = L::inf();
       if (rem > cap)
-        tbs = (tid < P.R ? g_rt : g_ct)[S.base[tid] + c + cap - 1];
+        tbs = (tid < P.R ? g_r : g_c)[S.base[tid] + c + cap - 1].t;
       atomicMin(&S.tb[phase & 1], L::bits(tbs));
     }
-    constexpr int W = LEAD; // leading records before the cursor
-    for (int idx = tid; idx < NS * (cap + W); idx += nthreads)
+    const int lane = tid & 63;
+    const int nwaves = nthreads >> 6;
+    for (int s = tid >> 6; s < NS; s += nwaves)
     {
-      const int s = idx / (cap + W);
-      const int j = idx - s * (cap + W);
-      const int64_t k = S.next[s] - W + j;
-      if (k >= 0 && k < S.m[s])
+      const int64_t m = wave_uniform(S.m[s]);
+      const int64_t first = wave_uniform(S.next[s]) - LEAD; // window record 0
+      // window records j in [jlo, jhi): record first + j exists
+      const int jlo = first < 0 ? 1 : 0;
+      const int jhi = static_cast<int>(m - first < cap + LEAD ? m - first : cap + LEAD);
+      const RecT* src = (s < P.R ? g_r : g_c) + wave_uniform(S.base[s]);
+      const uint32_t dst = wave_uniform(S.lbase[s]);
+      if constexpr (sizeof(RecT) == 16)
       {
-        const bool isrow = s < P.R;
-        const int64_t gk = S.base[s] + k;
-        Rec<TT, TV> r;
-        r.t = isrow ? g_rt[gk] : g_ct[gk];
-        r.v = isrow ? g_rv[gk] : g_cv[gk];
-        S.rec(S.lbase[s] + static_cast<uint32_t>(j) * S.lstride[s]) = r;
+        // Direct global -> LDS loads (global_load_lds_dwordx4): lane i of a
+        // 64-record chunk lands at chunk base + 16 i, which is exactly the
+        // window layout, so records bypass registers and every slot's loads
+        // are in flight together; one vmcnt wait below covers them.
+        for (int c0 = 0; c0 < jhi; c0 += 64)
+        {
+          const int j = c0 + lane;
+          if (j >= jlo && j < jhi)
+            lds_dma16(src + (first + j), S.smem + dst + 16u * static_cast<uint32_t>(c0));
+        }
+      }
+      else
+      {
+        for (int j = lane; j < jhi; j += 64)
+          if (j >= jlo)
+            S.rec(dst + static_cast<uint32_t>(j) * static_cast<uint32_t>(sizeof(RecT))) = src[first + j];
       }
     }
+    if constexpr (sizeof(RecT) == 16)
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     if (tid == 0)
       S.tb[(phase + 1) & 1] = L::bits(L::inf());
   }
