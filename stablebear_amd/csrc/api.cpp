@@ -894,9 +894,10 @@ extern "C"
       std::memset(&B, 0, sizeof(B));
       B.f = fwd_params(*plan, nullptr, out_dtype, nullptr);
       B.f.work_counter = nullptr;
-      // the accumulator slots double the LDS per staged record: a shorter
-      // phase keeps four workgroups resident per CU
-      B.f.cap = std::min(B.f.cap, plan->bwd_cap);      while (B.f.cap > 2 && bwd_lds_bytes(plan->tv_f64, B.f.R, B.f.C, B.f.cap) > plan->ctx->lds_per_block)
+      // the accumulator slots add LDS per staged record: a shorter phase
+      // keeps more workgroups resident per CU
+      B.f.cap = std::min(B.f.cap, plan->bwd_cap);
+      while (B.f.cap > 2 && bwd_lds_bytes(plan->tv_f64, B.f.R, B.f.C, B.f.cap) > plan->ctx->lds_per_block)
         B.f.cap /= 2;
       B.grad_out = grad_out_device;
       B.fwd_out = fwd_out_device;
