@@ -216,6 +216,16 @@ def main(argv=None):
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device-to-device copy (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
+    ap.add_argument("--batched", action="store_true",
+                    help="run only the batched leg: 256 independent pdist problems in one "
+                         "persistent, graph-captured, autotuned launch")
+    ap.add_argument("--batched-k", type=int, default=256)
+    ap.add_argument("--cdist", action="store_true",
+                    help="run only the large cdist leg (L1, N_X = N_Y = 50,000, "
+                         "m ~ U{500..5000}) with row means")
+    ap.add_argument("--cdist-n", type=int, default=50000)
+    ap.add_argument("--cdist-rows", type=int, default=48,
+                    help="rows of the cdist leg checked against the reference walk on the CPU")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -238,6 +248,12 @@ def main(argv=None):
         return 0
     if args.only_fwdbwd:
         print(json.dumps({"fwd_bwd": _fwd_bwd_c4(stream)}), flush=True)
+        return 0
+    if args.batched:
+        print(json.dumps(_batched_leg(args, stream)), flush=True)
+        return 0
+    if args.cdist:
+        print(json.dumps(_cdist_leg(args, stream)), flush=True)
         return 0
 
     n = args.n
@@ -397,6 +413,130 @@ def _fwd_bwd_c4(stream):
     fb = e1.elapsed_time(e2) / reps
     return {"workload": "L2 pdist N=4096 m~U{20..200}, loss sum(W*D)", "fwd_ms": f,
             "fwd_bwd_ms": fb, "ratio": fb / f}
+
+
+def _batched_leg(args, stream):
+    """256 independent pdist problems (N_b ~ U{64..4096}, m ~ round(LogNormal(4, 1))
+    clipped to [5, 20000], integer values in [-50, 50] ending at 0), L1, FP32C: one
+    persistent launch over all problems, tile shape picked by the autotuner, captured
+    in a hipGraph and replayed. Errors and the CPU rate on the smallest problems."""
+    import torch
+
+    import oracle
+    from stablebear_amd import workloads
+    from stablebear_amd.batched import BatchedPlan
+
+    t0 = time.perf_counter()
+    probs = workloads.c3(k=args.batched_k)
+    _progress(f"batched: {len(probs)} problems generated in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    bp = BatchedPlan(probs, op="l1", precision="fp32c")
+    plan = bp.plan
+    setup_s = time.perf_counter() - t0
+    out = torch.empty(bp.out_count, dtype=torch.float32, device="cuda")
+    shape, tune_ms = plan.autotune(out.data_ptr(), False, reps=1)
+    _progress(f"batched: autotuned tile {shape} ({tune_ms:.1f} ms)")
+    g = bp.graph(out.data_ptr(), False)
+    ms_graph, _ = _events_ms(lambda: g.launch(stream.cuda_stream), args.steps, args.warmup, stream)
+    ms_eager, _ = _events_ms(lambda: bp.execute(out.data_ptr(), False, stream.cuda_stream),
+                             args.steps, 1, stream)
+    _progress(f"batched: graph {ms_graph:.2f} ms, eager {ms_eager:.2f} ms")
+    g.launch(stream.cuda_stream)
+    torch.cuda.synchronize()
+    parts = bp.split(out.cpu().numpy())
+
+    # reference walk on the smallest problems: error of every entry, CPU rate
+    cores = len(os.sched_getaffinity(0))
+    use_ref = oracle.ref_available()
+    order = sorted(range(len(probs)), key=lambda b: (len(probs[b][0]) - 1) * int(probs[b][0][-1]))
+    cpu_pairs, cpu_s, max_rel, checked = 0, 0.0, 0.0, 0
+    for b in order:
+        off, t, v = probs[b]
+        t0 = time.perf_counter()
+        exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_PDIST, off, t, v, nthreads=cores,
+                              use_ref=use_ref)
+        cpu_s += time.perf_counter() - t0
+        cpu_pairs += exp.size
+        rel, _ = _errors(parts[b].astype(np.float64), exp)
+        max_rel = max(max_rel, rel)
+        checked += 1
+        if cpu_s > 10.0:
+            break
+    return {
+        "leg": "batched",
+        "workload": f"{len(probs)} independent L1 pdist problems, N_b ~ U{{64..4096}}, "
+                    "m ~ round(LogNormal(4, 1)) clipped to [5, 20000], integer values, FP32C",
+        "problems": len(probs), "pcfs": int(sum(bp.sizes)), "pairs": plan.pairs,
+        "segments": plan.segments, "tile": list(shape),
+        "graph_ms": ms_graph, "eager_ms": ms_eager,
+        "pairs_per_s": plan.pairs / (ms_graph * 1e-3),
+        "segments_per_s": plan.segments / (ms_graph * 1e-3),
+        "setup_s": setup_s,
+        "max_rel_err_vs_fp64_reference": max_rel,
+        "checked_problems": checked,
+        "cpu_baseline": {"value": cpu_pairs / cpu_s if cpu_s else None, "unit": "pairs/s",
+                         "cores": cores, "kind": "reference" if use_ref else "port",
+                         "sample": f"the {checked} smallest problems ({cpu_pairs} pairs, "
+                                   f"{cpu_s:.1f} s)"},
+    }
+
+
+def _cdist_leg(args, stream):
+    """L1 cdist of two sets of N PCFs (m ~ U{500..5000}, Exp increments of mean 10,
+    N(0,1) values ending at 0), FP32C with FP64 per-phase folding on long pairs,
+    dense output plus per-row sums (row means of D) in the same launch."""
+    import torch
+
+    import oracle
+    from stablebear_amd import runtime, workloads
+
+    n = args.cdist_n
+    t0 = time.perf_counter()
+    X, Y = workloads.c5(n)
+    _progress(f"cdist: generated 2 x {n} PCFs ({int(X[0][-1]) + int(Y[0][-1])} points) "
+              f"in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    sx = runtime.DeviceSet(runtime.csr_from_arrays(*X), "fp32c")
+    sy = runtime.DeviceSet(runtime.csr_from_arrays(*Y), "fp32c")
+    plan = runtime.Plan(sx, sy, op="l1")
+    setup_s = time.perf_counter() - t0
+    out = torch.empty(plan.out_count, dtype=torch.float32, device="cuda")
+    rs = torch.empty(n, dtype=torch.float64, device="cuda")
+    _progress(f"cdist: {plan.tiles} tiles, {plan.segments} segments; running")
+    ms, _ = _events_ms(lambda: plan.execute(out.data_ptr(), False, rs.data_ptr(),
+                                            stream.cuda_stream), 1, 0, stream)
+    _progress(f"cdist: {ms:.1f} ms")
+    row_means = rs.cpu().numpy() / n
+
+    # reference walk on the first rows against all columns (bounded sample)
+    cores = len(os.sched_getaffinity(0))
+    use_ref = oracle.ref_available()
+    r1 = max(1, min(args.cdist_rows, n))
+    xo, xt, xv = X
+    sub = (xo[:r1 + 1], xt[:int(xo[r1])], xv[:int(xo[r1])])
+    t0 = time.perf_counter()
+    exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_CDIST, *sub, *Y, nthreads=cores,
+                          use_ref=use_ref).reshape(r1, n)
+    cpu_s = time.perf_counter() - t0
+    got = out[: r1 * n].double().cpu().numpy().reshape(r1, n)
+    rel, _ = _errors(got.ravel(), exp.ravel())
+    mean_rel = float(np.max(np.abs(row_means[:r1] - exp.mean(axis=1)) / np.abs(exp.mean(axis=1))))
+    return {
+        "leg": "cdist",
+        "workload": f"L1 cdist {n} x {n}, m ~ U{{500..5000}}, Exp increments of mean 10, "
+                    "N(0,1) values ending at 0, FP32C + FP64 fold, dense output + row means",
+        "pairs": plan.pairs, "segments": plan.segments, "tiles": plan.tiles,
+        "tile": list(plan.shape), "ms": ms,
+        "pairs_per_s": plan.pairs / (ms * 1e-3),
+        "segments_per_s": plan.segments / (ms * 1e-3),
+        "setup_s": setup_s,
+        "max_rel_err_vs_fp64_reference": rel,
+        "row_mean_max_rel_err": mean_rel,
+        "cpu_baseline": {"value": exp.size / cpu_s, "unit": "pairs/s", "cores": cores,
+                         "kind": "reference" if use_ref else "port",
+                         "sample": f"rows 0..{r1 - 1} against all {n} columns ({exp.size} pairs, "
+                                   f"{cpu_s:.1f} s)"},
+    }
 
 
 if __name__ == "__main__":

@@ -38,6 +38,9 @@ namespace sbamd
     // them out in launch order) or, for persistent workgroups, from an atomic
     // counter over the longest-first tile list.
     uint32_t* const s_next_tile = reinterpret_cast<uint32_t*>(smem + lay.misc_off);
+    // row sums (when requested): the tile's R partial sums, one global atomic
+    // per row and tile instead of one per pair
+    double* const s_rows = reinterpret_cast<double*>(smem + lay.total);
     int64_t ti = blockIdx.x;
     for (;; ti += gridDim.x)
     {
@@ -55,6 +58,8 @@ namespace sbamd
       const bool fold = (tl.flags & TILE_FOLD) != 0;
 
       tile_begin(S, P, tl, pr, tid);
+      if (P.row_sum && tid < P.R)
+        s_rows[tid] = 0.0;
       __syncthreads();
 
       // Pair of this lane (sorted positions a = r0 + lr, b = c0 + lc).
@@ -181,22 +186,25 @@ namespace sbamd
         else
           static_cast<float*>(P.out)[idx] = static_cast<float>(res);
         if (P.row_sum)
-          atomicAdd(&P.row_sum[i], res);
+          atomicAdd(&s_rows[lr], res);
       }
       __syncthreads();
+      if (P.row_sum && tid < tl.nr)
+        atomicAdd(&P.row_sum[P.r_perm[pr.row_base + tl.r0 + tid]], s_rows[tid]);
     }
   }
 
+  // LDS of the forward kernel: the staging layout plus R row-sum partials.
   size_t fwd_lds_bytes(int tt_f64, int tv_f64, int R, int C, int cap)
   {
     (void)tv_f64;
-    return static_cast<size_t>(lds_layout(tt_f64 ? 8 : 4, R, C, cap, 0).total);
+    return static_cast<size_t>(lds_layout(tt_f64 ? 8 : 4, R, C, cap, 0).total) + 8 * static_cast<size_t>(R);
   }
 
   template <typename TT, typename TV, bool KAHAN>
   static void launch_typed(const FwdParams& prm, int op, hipStream_t stream)
   {
-    const size_t lds = static_cast<size_t>(lds_layout(sizeof(TT), prm.R, prm.C, prm.cap, 0).total);
+    const size_t lds = fwd_lds_bytes(sizeof(TT) == 8, 0, prm.R, prm.C, prm.cap);
     if (prm.ntiles <= 0)
       return;
     const unsigned threads = tile_block_threads(prm.R, prm.C);
