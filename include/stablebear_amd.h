@@ -97,6 +97,8 @@ typedef struct
   uint64_t input_bytes;    /* device bytes of the staged record arrays */
   int32_t rows, cols, cap; /* tile shape in use */
   int32_t problems;        /* 1, or k for a batched plan */
+  int32_t long_pairs;      /* 1 if the plan runs the long-pair kernel */
+  int32_t reserved;
 } sbamd_plan_stats;
 
 /* ---- status / device ---- */
@@ -149,6 +151,16 @@ int sbamd_plan_set_fold_threshold(sbamd_plan* plan, int64_t segments);
  * fill kernel resets before each launch (graph-capture safe). Off: one
  * workgroup per tile. */
 int sbamd_plan_set_persistent(sbamd_plan* plan, int32_t on);
+/* Long-pair kernel: one wavefront per pair, the pair's union segments split
+ * across the 64 lanes at merge-path points, partial sums added in FP64. For
+ * plans of few, long pairs (a small N of long PCFs, lp_distance of long
+ * PCFs), where one lane per pair cannot fill the device. mode 0: never;
+ * 1: always (FP64 results then differ from the sequential sum in the last
+ * bits); 2 (default): FP32C plans with fewer than ~384 pairs per CU and
+ * >= 2048 segments per pair on average. Not for batched plans. No
+ * reference counterpart (the reference walks every pair on one thread,
+ * cuda_pcf_kernel.cuh:95-136). */
+int sbamd_plan_set_long_pairs(sbamd_plan* plan, int32_t mode);
 
 /* Enqueue the whole forward computation on `stream` (a hipStream_t; NULL is
  * the null stream, as in HIP). out_device: device buffer of plan out_count elements of
@@ -190,7 +202,9 @@ int sbamd_graph_launch(sbamd_graph* graph, void* stream);
 int sbamd_graph_destroy(sbamd_graph* graph);
 
 /* ---- autotuner: times candidate tile shapes on this plan and keeps the best
- * (also returned). Candidates are limited by the LDS budget. ---- */
+ * (also returned). Candidates are limited by the LDS budget. For FP32C
+ * plans (or plans set to long-pair mode 1) the long-pair kernel is a
+ * candidate too; best->reserved is 1 when it won (the plan then uses it). ---- */
 int sbamd_autotune(sbamd_plan* plan, void* out_device, int32_t out_dtype, int32_t reps,
                    sbamd_tile_shape* best, double* best_ms);
 int sbamd_plan_set_shape(sbamd_plan* plan, const sbamd_tile_shape* shape);

@@ -56,7 +56,8 @@ class PlanStats(ctypes.Structure):
     _fields_ = [("pairs", ctypes.c_uint64), ("segments", ctypes.c_uint64),
                 ("out_count", ctypes.c_uint64), ("tiles", ctypes.c_uint64),
                 ("input_bytes", ctypes.c_uint64), ("rows", ctypes.c_int32),
-                ("cols", ctypes.c_int32), ("cap", ctypes.c_int32), ("problems", ctypes.c_int32)]
+                ("cols", ctypes.c_int32), ("cap", ctypes.c_int32), ("problems", ctypes.c_int32),
+                ("long_pairs", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 # name -> (restype, argtypes); restype None means int status
@@ -82,6 +83,7 @@ _SIGS = {
     "sbamd_plan_out_offset": (None, [vp, ctypes.c_int32, c_u64p]),
     "sbamd_plan_set_fold_threshold": (None, [vp, ctypes.c_int64]),
     "sbamd_plan_set_persistent": (None, [vp, ctypes.c_int32]),
+    "sbamd_plan_set_long_pairs": (None, [vp, ctypes.c_int32]),
     "sbamd_plan_execute": (None, [vp, vp, ctypes.c_int32, vp, vp]),
     "sbamd_plan_backward": (None, [vp, vp, vp, ctypes.c_int32, vp, vp, ctypes.c_int32, vp]),
     "sbamd_plan_execute_to_host": (None, [vp, vp, ctypes.c_int32]),

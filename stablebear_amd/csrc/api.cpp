@@ -279,9 +279,57 @@ namespace sbamd
       return P;
     }
 
+    // Work units of a plan's forward launch: pairs for the long-pair kernel,
+    // tiles otherwise. The job path runs a plan in unit ranges.
+    int64_t work_units(const Plan& pl)
+    {
+      return pl.use_long ? static_cast<int64_t>(pl.pairs) : static_cast<int64_t>(pl.h_tiles.size());
+    }
+
+    // Long-pair kernel on a plan of few, long pairs (FP32C), or when asked.
+    // One lane per pair fills the device only with ~1536 pairs in flight per
+    // CU; below a quarter of that, with >= 2048 segments per pair on
+    // average, a wavefront per pair is the better split.
+    void decide_long(Plan& pl)
+    {
+      if (pl.batched || pl.pairs == 0)
+        pl.use_long = false;
+      else if (pl.long_mode == 1)
+        pl.use_long = true;
+      else if (pl.long_mode == 0)
+        pl.use_long = false;
+      else
+        pl.use_long = pl.precision == SBAMD_PREC_FP32C &&
+                      pl.pairs * 4 <= static_cast<uint64_t>(pl.ctx->cu_count) * 1536 &&
+                      pl.segments >= pl.pairs * 2048;
+    }
+
     void execute_range(const Plan& pl, void* out, int32_t out_dtype, double* row_sum, int64_t t0,
                        int64_t t1, hipStream_t stream)
     {
+      if (pl.use_long)
+      {
+        const FwdParams F = fwd_params(pl, out, out_dtype, row_sum);
+        LongParams L;
+        std::memset(&L, 0, sizeof(L));
+        L.r_off = F.r_off;
+        L.r_rec = F.r_rec;
+        L.r_perm = F.r_perm;
+        L.c_off = F.c_off;
+        L.c_rec = F.c_rec;
+        L.c_perm = F.c_perm;
+        L.out = out;
+        L.row_sum = row_sum;
+        L.q0 = t0;
+        L.q1 = t1;
+        L.n_cols = pl.cdist ? pl.sets[1]->n : 0;
+        L.p = pl.p;
+        L.out_kind = pl.out_kind;
+        L.out_f64 = out_dtype == SBAMD_F64 ? 1 : 0;
+        launch_long(L, pl.tt_f64, pl.tv_f64, kernel_op(pl.op, pl.p),
+                    pl.precision == SBAMD_PREC_FP32C ? 1 : 0, pl.ctx->cu_count, stream);
+        return;
+      }
       if (fwd_lds_bytes(pl.tt_f64, pl.tv_f64, pl.R, pl.C, pl.cap) > pl.ctx->lds_per_block)
         fail(SBAMD_ERR_INVALID, "tile shape needs more LDS than a workgroup may use");
       FwdParams P = fwd_params(pl, out, out_dtype, row_sum);
@@ -591,6 +639,7 @@ extern "C"
         pl->C = pl->R;
       check_shape(pl->R, pl->C, pl->cap);
       pl->rebuild_tiles();
+      decide_long(*pl);
       *out = pl.release();
     });
   }
@@ -707,6 +756,21 @@ extern "C"
       info->cols = plan->C;
       info->cap = plan->cap;
       info->problems = static_cast<int32_t>(plan->h_probs.size());
+      info->long_pairs = plan->use_long ? 1 : 0;
+      info->reserved = 0;
+    });
+  }
+
+  int sbamd_plan_set_long_pairs(sbamd_plan* plan, int32_t mode)
+  {
+    return guarded([&] {
+      need(plan, "plan");
+      if (mode < 0 || mode > 2)
+        fail(SBAMD_ERR_INVALID, "long-pair mode must be 0 (off), 1 (on) or 2 (auto)");
+      if (mode == 1 && plan->batched)
+        fail(SBAMD_ERR_INVALID, "batched plans use the tile kernel");
+      plan->long_mode = mode;
+      decide_long(*plan);
     });
   }
 
@@ -773,7 +837,7 @@ extern "C"
       if (row_sum_device)
         launch_fill_zero(row_sum_device, static_cast<size_t>(plan->sets[0]->n) * sizeof(double), st);
       execute_range(*plan, out_device, out_dtype, static_cast<double*>(row_sum_device), 0,
-                    static_cast<int64_t>(plan->h_tiles.size()), st);
+                    work_units(*plan), st);
     });
   }
 
@@ -788,7 +852,7 @@ extern "C"
       use_device(plan->ctx);
       const size_t eb = out_dtype == SBAMD_F64 ? 8 : 4;
       DeviceBuffer dout(static_cast<size_t>(plan->out_count) * eb);
-      execute_range(*plan, dout.ptr, out_dtype, nullptr, 0, static_cast<int64_t>(plan->h_tiles.size()),
+      execute_range(*plan, dout.ptr, out_dtype, nullptr, 0, work_units(*plan),
                     plan->ctx->stream);
       SBAMD_HIP(hipStreamSynchronize(plan->ctx->stream));
       SBAMD_HIP(hipMemcpy(out_host, dout.ptr, static_cast<size_t>(plan->out_count) * eb,
@@ -944,7 +1008,7 @@ extern "C"
         try
         {
           execute_range(*plan, out_device, out_dtype, nullptr, 0,
-                        static_cast<int64_t>(plan->h_tiles.size()), cs);
+                        work_units(*plan), cs);
         }
         catch (...)
         {
@@ -1000,6 +1064,7 @@ extern "C"
       use_device(plan->ctx);
       hipStream_t st = plan->ctx->stream;
       const int R0 = plan->R, C0 = plan->C, cap0 = plan->cap;
+      plan->use_long = false; // tile candidates first
       struct Cand
       {
         int R, C, cap;
@@ -1051,11 +1116,11 @@ extern "C"
           plan->cap = c.cap;
           plan->rebuild_tiles();
           execute_range(*plan, out_device, out_dtype, nullptr, 0,
-                        static_cast<int64_t>(plan->h_tiles.size()), st); // warm-up
+                        work_units(*plan), st); // warm-up
           SBAMD_HIP(hipEventRecord(e0, st));
           for (int r = 0; r < reps; ++r)
             execute_range(*plan, out_device, out_dtype, nullptr, 0,
-                          static_cast<int64_t>(plan->h_tiles.size()), st);
+                          work_units(*plan), st);
           SBAMD_HIP(hipEventRecord(e1, st));
           SBAMD_HIP(hipEventSynchronize(e1));
           float ms = 0.f;
@@ -1074,18 +1139,51 @@ extern "C"
         (void)hipEventDestroy(e1);
         throw;
       }
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
       plan->R = bc.R;
       plan->C = bc.C;
       plan->cap = bc.cap;
       plan->rebuild_tiles();
+      // the long-pair kernel is a candidate too (FP32C plans, or plans that
+      // asked for it)
+      bool chose_long = false;
+      if (!plan->batched && plan->pairs > 0 &&
+          (plan->precision == SBAMD_PREC_FP32C || plan->long_mode == 1))
+      {
+        plan->use_long = true;
+        try
+        {
+          execute_range(*plan, out_device, out_dtype, nullptr, 0, work_units(*plan), st);
+          SBAMD_HIP(hipEventRecord(e0, st));
+          for (int r = 0; r < reps; ++r)
+            execute_range(*plan, out_device, out_dtype, nullptr, 0, work_units(*plan), st);
+          SBAMD_HIP(hipEventRecord(e1, st));
+          SBAMD_HIP(hipEventSynchronize(e1));
+        }
+        catch (...)
+        {
+          (void)hipEventDestroy(e0);
+          (void)hipEventDestroy(e1);
+          throw;
+        }
+        float ms = 0.f;
+        SBAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const double per = static_cast<double>(ms) / reps;
+        if (per < bestv)
+        {
+          bestv = per;
+          chose_long = true;
+        }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      plan->use_long = chose_long;
+      plan->long_mode = chose_long ? 1 : 0;
       if (best)
       {
         best->rows = bc.R;
         best->cols = bc.C;
         best->cap = bc.cap;
-        best->reserved = 0;
+        best->reserved = chose_long ? 1 : 0;
       }
       if (best_ms)
         *best_ms = bestv;
@@ -1146,7 +1244,7 @@ extern "C"
             hipStream_t st;
             SBAMD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
             std::unique_ptr<std::remove_pointer<hipStream_t>::type, decltype(&hipStreamDestroy)> sguard(st, &hipStreamDestroy);
-            const int64_t nt = static_cast<int64_t>(plan->h_tiles.size());
+            const int64_t nt = work_units(*plan);
             const int64_t chunks = std::min<int64_t>(nt, 64);
             const bool diag = out_kind == SBAMD_OUT_SYMMAT;
             hipEvent_t ev;
@@ -1167,8 +1265,11 @@ extern "C"
               }
               SBAMD_HIP(hipStreamSynchronize(st));
               uint64_t done = 0;
-              for (int64_t k = t0; k < t1; ++k)
-                done += tile_pairs(plan->h_tiles[k], diag);
+              if (plan->use_long)
+                done = static_cast<uint64_t>(t1 - t0);
+              else
+                for (int64_t k = t0; k < t1; ++k)
+                  done += tile_pairs(plan->h_tiles[k], diag);
               jp->progress += done;
             }
             SBAMD_HIP(hipMemcpy(out_host, dout.ptr, static_cast<size_t>(plan->out_count) * eb,
@@ -1268,7 +1369,7 @@ extern "C"
         fail(rc, g_last_error);
       std::unique_ptr<sbamd_plan> plan(pl);
       DeviceBuffer dout(sizeof(double));
-      execute_range(*plan, dout.ptr, SBAMD_F64, nullptr, 0, static_cast<int64_t>(plan->h_tiles.size()),
+      execute_range(*plan, dout.ptr, SBAMD_F64, nullptr, 0, work_units(*plan),
                     ctx->stream);
       SBAMD_HIP(hipStreamSynchronize(ctx->stream));
       double r = 0.0;

@@ -85,6 +85,24 @@ namespace sbamd
     uint32_t* work_counter; // non-null: persistent workgroups take tiles from this counter
   };
 
+  // Kernel parameters of the long-pair kernel (one wavefront per pair).
+  struct LongParams
+  {
+    const int64_t* r_off;
+    const void* r_rec;
+    const int64_t* r_perm;
+    const int64_t* c_off;
+    const void* c_rec;
+    const int64_t* c_perm;
+    void* out;
+    double* row_sum;  // optional (DENSE): per original row sums
+    int64_t q0, q1;   // pair range [q0, q1) in the plan's pair order
+    int64_t n_cols;   // DENSE row length (cdist)
+    double p;
+    int32_t out_kind;
+    int32_t out_f64;
+  };
+
   struct BwdParams
   {
     FwdParams f;          // forward geometry (out unused)
@@ -202,6 +220,10 @@ namespace sbamd
     // backward accumulators (per record q, a; per PCF wsum), allocated on first use
     DeviceBuffer bwd_q, bwd_a, bwd_w;
     int32_t bwd_cap = 32; // records staged per slot and phase in the backward
+    // long-pair kernel: mode 0 off, 1 on, 2 auto (few long pairs, FP32C);
+    // use_long is the decision for this plan
+    int32_t long_mode = 2;
+    bool use_long = false;
     uint64_t input_bytes = 0;
     void rebuild_tiles();
   };
@@ -210,6 +232,8 @@ namespace sbamd
   size_t fwd_lds_bytes(int tt_f64, int tv_f64, int R, int C, int cap);
   void launch_forward(const FwdParams& prm, int tt_f64, int tv_f64, int op, int kahan,
                       hipStream_t stream);
+  void launch_long(const LongParams& prm, int tt_f64, int tv_f64, int op, int kahan, int cu_count,
+                   hipStream_t stream);
   void launch_backward(const BwdParams& prm, int op, hipStream_t stream);
   void launch_finalize_grad(const FinalizeParams& prm, hipStream_t stream);
   size_t bwd_lds_bytes(int tv_f64, int R, int C, int cap);

@@ -318,6 +318,13 @@ class Plan:
         self.input_bytes = st.input_bytes
         self.shape = (st.rows, st.cols, st.cap)
         self.problems = st.problems
+        self.long_pairs = bool(st.long_pairs)
+
+    def set_long_pairs(self, mode):
+        """Long-pair kernel: False / "off", True / "on", or "auto" (the default)."""
+        code = {False: 0, "off": 0, True: 1, "on": 1, "auto": 2}[mode]
+        L.check(L.lib().sbamd_plan_set_long_pairs(self.handle, code))
+        self.refresh_stats()
 
     def out_offset(self, problem: int) -> int:
         off = ctypes.c_uint64()
