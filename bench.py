@@ -226,6 +226,9 @@ def main(argv=None):
     ap.add_argument("--cdist-n", type=int, default=50000)
     ap.add_argument("--cdist-rows", type=int, default=48,
                     help="rows of the cdist leg checked against the reference walk on the CPU")
+    ap.add_argument("--long", action="store_true",
+                    help="run only the long-pair leg: L1 pdist of 64 PCFs with m ~ U{50k..100k}, "
+                         "tile kernel vs long-pair kernel")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -254,6 +257,9 @@ def main(argv=None):
         return 0
     if args.cdist:
         print(json.dumps(_cdist_leg(args, stream)), flush=True)
+        return 0
+    if args.long:
+        print(json.dumps(_long_leg(args, stream)), flush=True)
         return 0
 
     n = args.n
@@ -536,6 +542,53 @@ def _cdist_leg(args, stream):
                          "kind": "reference" if use_ref else "port",
                          "sample": f"rows 0..{r1 - 1} against all {n} columns ({exp.size} pairs, "
                                    f"{cpu_s:.1f} s)"},
+    }
+
+
+def _long_leg(args, stream):
+    """Few, long pairs: L1 pdist of 64 PCFs with m ~ U{50,000..100,000} (Exp(1)
+    increments, N(0,1) values ending at 0), FP32C. The tile kernel (one lane per
+    pair: 2016 pairs are 8 workgroups) against the long-pair kernel (one
+    wavefront per pair)."""
+    import torch
+
+    import oracle
+    from stablebear_amd import runtime, workloads
+
+    rng = np.random.default_rng(6006)
+    m = rng.integers(50_000, 100_001, size=64).astype(np.int64)
+    off, t = workloads._times(rng, m, 1.0)
+    v = workloads._normal_values(rng, off)
+    ds = runtime.DeviceSet(runtime.csr_from_arrays(off, t, v), "fp32c")
+    plan = runtime.Plan(ds, op="l1")
+    out = torch.empty(plan.out_count, dtype=torch.float32, device="cuda")
+    res = {}
+    for mode in ("off", "on"):
+        plan.set_long_pairs(mode)
+        ms, _ = _events_ms(lambda: plan.execute(out.data_ptr(), False, stream=stream.cuda_stream),
+                           max(1, min(args.steps, 3)), 1, stream)
+        res[mode] = ms
+        _progress(f"long: long-pair kernel {mode}: {ms:.2f} ms")
+    got = out.double().cpu().numpy()
+    cores = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_PDIST, off, t, v, nthreads=cores,
+                          use_ref=oracle.ref_available())
+    cpu_s = time.perf_counter() - t0
+    rel, _ = _errors(got, exp)
+    plan.set_long_pairs("auto")
+    return {
+        "leg": "long_pairs",
+        "workload": "L1 pdist, 64 PCFs, m ~ U{50000..100000}, FP32C",
+        "pairs": plan.pairs, "segments": plan.segments,
+        "tile_kernel_ms": res["off"], "long_kernel_ms": res["on"],
+        "speedup": res["off"] / res["on"],
+        "long_segments_per_s": plan.segments / (res["on"] * 1e-3),
+        "auto_choice_long": plan.long_pairs,
+        "max_rel_err_vs_fp64_reference": rel,
+        "cpu_baseline": {"value": plan.pairs / cpu_s, "unit": "pairs/s", "cores": cores,
+                         "kind": "reference" if oracle.ref_available() else "port",
+                         "sample": f"the whole pdist ({plan.pairs} pairs, {cpu_s:.1f} s)"},
     }
 
 
