@@ -39,7 +39,11 @@ namespace sbamd
     const TileLds<TT, TV> S(smem, lay);
     static_assert(sizeof(RecT) == 16, "accumulator slots mirror 16-byte records");
     constexpr uint32_t REC = sizeof(RecT);
-    const uint32_t GOFF = static_cast<uint32_t>(lay.extra_off);
+    // Q and A are two arrays of one FP64 per staged record (8-byte stride,
+    // so the lanes' atomics of one instruction spread over all LDS banks);
+    // record at LDS byte address x has its slots at QOFF / AOFF + x / 2.
+    const uint32_t QOFF = static_cast<uint32_t>(lay.extra_off);
+    const uint32_t AOFF = QOFF + static_cast<uint32_t>(lay.rec_end / 2);
     double* const s_w = reinterpret_cast<double*>(smem + lay.total);
 
     const int tid = threadIdx.x;
@@ -110,7 +114,7 @@ namespace sbamd
       {
         phase_stage<TT, TV, 1>(S, P, phase, tid, nthreads);
         for (int k = tid; k < lay.rec_end / 16; k += nthreads)
-          *reinterpret_cast<double2*>(smem + GOFF + 16 * k) = make_double2(0.0, 0.0);
+          *reinterpret_cast<double2*>(smem + QOFF + 16 * k) = make_double2(0.0, 0.0);
         __syncthreads();
         const TT Tb = L::from_bits(S.tb[phase & 1]);
         phase_advance_cursor<TT, TV, 1>(S, P, Tb, tid);
@@ -140,9 +144,9 @@ namespace sbamd
             hi_acc = fma(lo_w, len, hi_acc);
             t = lo_t;
             // lo leaves its current record (the one before lo_p)
-            double* const slot = reinterpret_cast<double*>(smem + GOFF + (lo_p - REC));
-            atomicAdd(slot, lo_acc);
-            atomicAdd(slot + 1, hi_w);
+            const uint32_t half = (lo_p - REC) >> 1;
+            atomicAdd(reinterpret_cast<double*>(smem + QOFF + half), lo_acc);
+            atomicAdd(reinterpret_cast<double*>(smem + AOFF + half), hi_w);
             const RecT r = S.rec(lo_p);
             const double rw = w * static_cast<double>(r.v);
             const uint32_t np = lo_p + REC;
@@ -178,12 +182,14 @@ namespace sbamd
           const int64_t k = S.cur[s] - 1 + j;
           if (k >= 0 && k < S.next[s])
           {
-            const double2 qa = *reinterpret_cast<const double2*>(smem + GOFF + S.lbase[s] + j * REC);
+            const uint32_t half = (S.lbase[s] + static_cast<uint32_t>(j) * REC) >> 1;
+            const double q = *reinterpret_cast<const double*>(smem + QOFF + half);
+            const double a = *reinterpret_cast<const double*>(smem + AOFF + half);
             const int64_t g = S.base[s] + k;
-            if (qa.x != 0.0)
-              atomicAdd(&B.q[g], qa.x);
-            if (qa.y != 0.0)
-              atomicAdd(&B.a[g], qa.y);
+            if (q != 0.0)
+              atomicAdd(&B.q[g], q);
+            if (a != 0.0)
+              atomicAdd(&B.a[g], a);
           }
         }
         __syncthreads();
