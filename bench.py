@@ -181,6 +181,12 @@ def _other_ops(csr, off, t, v, n, steps, stream, cores, sample_pairs):
         idx = _storage_rows(n, r1, diag)
         exp = exp[idx]
         entry = {"sample_rows": r1}
+        scale = None
+        if diag:
+            # the inner product is a signed sum: its error is stated relative
+            # to int |f| |g| (the condition of the sum) as well as entrywise
+            scale = oracle.pairwise(oracle.OP_DOT, 1.0, kind, off, t, np.abs(v), nthreads=cores,
+                                    rows=(0, r1))[idx]
         for prec in ("fp32c", "fp64"):
             ds = runtime.DeviceSet(csr, prec)
             plan = runtime.Plan(ds, op=op, p=p)  # "dot" -> symmetric matrix with diagonal
@@ -194,6 +200,9 @@ def _other_ops(csr, off, t, v, n, steps, stream, cores, sample_pairs):
             rel, normwise = _errors(got, exp)
             entry[prec] = {"ms": ms, "pairs_per_s": plan.pairs / (ms * 1e-3),
                            "max_rel_err": rel, "normwise_err": normwise}
+            if scale is not None:
+                entry[prec]["max_err_rel_to_abs_integral"] = float(
+                    np.max(np.abs(got - exp) / scale))
             del out, plan, ds
         entry["fp32c_over_fp64"] = entry["fp64"]["ms"] / entry["fp32c"]["ms"]
         res[name] = entry
