@@ -127,6 +127,14 @@ int sbamd_set_info(const sbamd_set* set, int64_t* n, int64_t* records, int64_t* 
 /* Replace the values of an existing set from a device array laid out like the
  * original v (offsets[n] entries, set dtype); times and structure unchanged. */
 int sbamd_set_update_values(sbamd_set* set, const void* v_device, void* stream);
+/* L_p norm of every PCF of a device set, (integral over [0, MAX) of |f|^p)^(1/p)
+ * with MAX the largest finite value of the set's time type; p >= 1. Writes n
+ * values of out_dtype (SBAMD_F32 / SBAMD_F64) to out_device in the original
+ * PCF order, asynchronously on `stream` (NULL: the legacy default stream).
+ * Replaces l1_norm / lp_norm applied per element by ApplyFunctional
+ * (include/sbear/algorithms/functional/apply_functional.hpp:63-100,
+ * src/python/functional/py_norms.cpp:24-46). */
+int sbamd_set_lp_norm(sbamd_set* set, double p, void* out_device, int32_t out_dtype, void* stream);
 
 /* ---- plans: tile work list for one pairwise operation ----
  * cols == NULL: pdist (DISTMAT) or l2_kernel (SYMMAT) over rows;
@@ -229,6 +237,12 @@ int sbamd_job_destroy(sbamd_job* job);
 /* lp_distance(f, g, p): lp_distance.hpp:10-21. f, g: one-PCF sets. */
 int sbamd_lp_distance(sbamd_ctx* ctx, const sbamd_pcf_set* f, const sbamd_pcf_set* g, double p,
                       double* out);
+/* lp_norm(fs, p) of stablebear/norms.py:10-53 (backends Norms_f32_f32 /
+ * Norms_f64_f64, src/python/functional/py_norms.cpp:24-68): uploads the host
+ * set in the layout of `precision`, runs sbamd_set_lp_norm and copies the n
+ * norms (out_dtype) to out_host; synchronous. */
+int sbamd_lp_norm(sbamd_ctx* ctx, const sbamd_pcf_set* set, double p, int32_t precision,
+                  void* out_host, int32_t out_dtype);
 /* Rectangles of the walk over [a, b): iterate_rectangles.hpp:37-104. out
  * holds 4 values (left, right, f_value, g_value) per rectangle in the set
  * dtype; *count receives the total (only the first cap are written). */

@@ -209,6 +209,23 @@ def pairwise(op, p, kind, off, t, v, off2=None, t2=None, v2=None, dtype=np.float
     return out
 
 
+def lp_norms(p, off, t, v, dtype=np.float64, use_ref=False):
+    """L_p norm of each PCF of a CSR set over [0, MAX) (apply_functional.hpp:63-100)."""
+    dtype = np.dtype(dtype)
+    off, t, v = _csr(off, t, v, dtype)
+    n = len(off) - 1
+    out = np.zeros(n, dtype=dtype)
+    lib = _load_ref() if use_ref else _lib
+    suf = _suffix(dtype)
+    fp = _f64p if suf == "f64" else _f32p
+    ft = ctypes.c_double if suf == "f64" else ctypes.c_float
+    fn = getattr(lib, f"{'ref' if use_ref else 'orc'}_lp_norms_{suf}")
+    fn.restype = None
+    fn.argtypes = [ft, ctypes.c_long, _i64p, fp, fp, fp]
+    fn(p, n, _ptr(off, _i64p), _ptr(t, fp), _ptr(v, fp), _ptr(out, fp))
+    return out
+
+
 def grad(op, kind, off, t, v, G, D=None, nthreads=None):
     """Analytic (dL/dv, dL/dt) of L = sum_pairs G * X (X = L2 distance or inner product)."""
     off, t, v = _csr(off, t, v, np.float64)

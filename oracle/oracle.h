@@ -57,6 +57,12 @@ float orc_integrate_f32(int op, float p,
                         const float* gt, const float* gv, long mg,
                         float a, float b);
 
+/* L_p norms of each PCF of a CSR set over [0, MAX) (apply_functional.hpp:63-100). */
+void orc_lp_norms_f64(double p, long n, const int64_t* off, const double* t, const double* v,
+                      double* out);
+void orc_lp_norms_f32(float p, long n, const int64_t* off, const float* t, const float* v,
+                      float* out);
+
 double orc_finalize_f64(int op, double p, double x);
 float orc_finalize_f32(int op, float p, float x);
 

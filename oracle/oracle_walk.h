@@ -99,6 +99,42 @@ long SUF(orc_rects)(const T* ft, const T* fv, long mf,
   return c.count;
 }
 
+/* Norm of one PCF over [0, MAX): l1_norm / lp_norm of
+ * include/sbear/algorithms/functional/apply_functional.hpp:63-100 over
+ * iterate_segments (iterate_rectangles.hpp:106-155): from the last point at
+ * or before 0, segments [t_k, min(t_{k+1}, MAX)) with value v_k, then the
+ * last point's segment to MAX; out = fma(r - l, |v| (p = 1) or |v|^p, out);
+ * the result is out (p = 1) or out^(1/p). An empty PCF has norm 0. */
+T SUF(orc_lp_norm)(T p, const T* t, const T* v, long m)
+{
+  if (m == 0)
+    return (T)0;
+  const T b = TMAX;
+  const int l1 = p == (T)1;
+  long k = SUF(start_index)(t, m, (T)0);
+  T tprev = 0;
+  T vprev = v[k];
+  T out = 0;
+  for (long i = k + 1; i < m; ++i)
+  {
+    T tnext = t[i] < b ? t[i] : b;
+    out = FMA(tnext - tprev, l1 ? FABS(vprev) : POW(FABS(vprev), p), out);
+    if (tnext >= b)
+      return l1 ? out : POW(out, (T)1 / p);
+    tprev = t[i];
+    vprev = v[i];
+  }
+  out = FMA(b - tprev, l1 ? FABS(vprev) : POW(FABS(vprev), p), out);
+  return l1 ? out : POW(out, (T)1 / p);
+}
+
+/* Norms of a CSR set: out[i] = norm of PCF i. */
+void SUF(orc_lp_norms)(T p, long n, const int64_t* off, const T* t, const T* v, T* out)
+{
+  for (long i = 0; i < n; ++i)
+    out[i] = SUF(orc_lp_norm)(p, t + off[i], v + off[i], (long)(off[i + 1] - off[i]));
+}
+
 /* Pair ops, operations.cuh:19-65. */
 static inline T SUF(pair_op)(int op, T p, T x, T y)
 {

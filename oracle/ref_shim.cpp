@@ -19,6 +19,7 @@
 #include <sbear/functional/operations.cuh>
 
 #include <atomic>
+#include <cmath>
 #include <cstdint>
 #include <limits>
 #include <thread>
@@ -59,6 +60,26 @@ namespace
       val += (rect.right - rect.left) * op(rect.f_value, rect.g_value);
     }, T(0), std::numeric_limits<T>::max());
     return val;
+  }
+
+  // l1_norm / lp_norm (apply_functional.hpp:63-100, which needs taskflow):
+  // their accumulation over the reference's own iterate_segments.
+  template <typename T>
+  T lp_norm(const Points<T>& f, T p)
+  {
+    using seg_t = sb::Segment<T, T>;
+    T out = T(0);
+    if (p == T(1))
+    {
+      sb::iterate_segments(f.cbegin(), f.cend(), T(0), std::numeric_limits<T>::infinity(),
+                           [&out](const seg_t& seg) { out += (seg.right - seg.left) * std::abs(seg.value); });
+      return out;
+    }
+    sb::iterate_segments(f.cbegin(), f.cend(), T(0), std::numeric_limits<T>::infinity(),
+                         [&out, p](const seg_t& seg) {
+                           out += (seg.right - seg.left) * std::pow(std::abs(seg.value), p);
+                         });
+    return std::pow(out, T(1) / p);
   }
 
   template <typename T, typename Op>
@@ -174,5 +195,19 @@ extern "C"
                        const float* v2, float* out, int nthreads)
   {
     return pairwise<float>(op, p, kind, n, off, t, v, n2, off2, t2, v2, 0, n, out, nthreads);
+  }
+
+  void ref_lp_norms_f64(double p, long n, const int64_t* off, const double* t, const double* v,
+                        double* out)
+  {
+    for (long i = 0; i < n; ++i)
+      out[i] = lp_norm<double>(make_points(t + off[i], v + off[i], static_cast<long>(off[i + 1] - off[i])), p);
+  }
+
+  void ref_lp_norms_f32(float p, long n, const int64_t* off, const float* t, const float* v,
+                        float* out)
+  {
+    for (long i = 0; i < n; ++i)
+      out[i] = lp_norm<float>(make_points(t + off[i], v + off[i], static_cast<long>(off[i + 1] - off[i])), p);
   }
 }

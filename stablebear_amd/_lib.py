@@ -73,6 +73,7 @@ _SIGS = {
     "sbamd_set_destroy": (None, [vp]),
     "sbamd_set_info": (None, [vp, c_i64p, c_i64p, c_i64p]),
     "sbamd_set_update_values": (None, [vp, vp, vp]),
+    "sbamd_set_lp_norm": (None, [vp, ctypes.c_double, vp, ctypes.c_int32, vp]),
     "sbamd_plan_create": (None, [vp, vp, vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
                                  ctypes.POINTER(TileShape), ctypes.POINTER(vp)]),
     "sbamd_plan_create_batched": (None, [vp, ctypes.c_int32, ctypes.POINTER(vp), ctypes.c_int32,
@@ -105,6 +106,8 @@ _SIGS = {
     "sbamd_job_destroy": (None, [vp]),
     "sbamd_lp_distance": (None, [vp, ctypes.POINTER(PcfSet), ctypes.POINTER(PcfSet),
                                  ctypes.c_double, c_dblp]),
+    "sbamd_lp_norm": (None, [vp, ctypes.POINTER(PcfSet), ctypes.c_double, ctypes.c_int32, vp,
+                             ctypes.c_int32]),
     "sbamd_iterate_rectangles": (None, [vp, ctypes.POINTER(PcfSet), ctypes.POINTER(PcfSet),
                                         ctypes.c_double, ctypes.c_double, vp, ctypes.c_int64,
                                         c_i64p]),

@@ -569,6 +569,29 @@ extern "C"
     });
   }
 
+  int sbamd_set_lp_norm(sbamd_set* set, double p, void* out_device, int32_t out_dtype, void* stream)
+  {
+    return guarded([&] {
+      need(set, "set");
+      check_out_dtype(out_dtype);
+      if (!(p >= 1.0))
+        fail(SBAMD_ERR_INVALID, "p must be >= 1.");
+      if (set->n == 0)
+        return;
+      need(out_device, "out_device");
+      use_device(set->ctx);
+      NormParams prm{};
+      prm.roff = set->roff.as<int64_t>();
+      prm.rec = set->rec.ptr;
+      prm.perm = set->perm.as<int64_t>();
+      prm.n = set->n;
+      prm.p = p;
+      prm.out = out_device;
+      prm.out_f64 = out_dtype == SBAMD_F64;
+      launch_norm(prm, set->tt_f64, set->tv_f64, pick_stream(set->ctx, stream));
+    });
+  }
+
   int sbamd_plan_create(sbamd_ctx* ctx, const sbamd_set* rows, const sbamd_set* cols, int32_t op,
                         double p, int32_t out_kind, const sbamd_tile_shape* shape, sbamd_plan** out)
   {
@@ -1375,6 +1398,31 @@ extern "C"
       double r = 0.0;
       SBAMD_HIP(hipMemcpy(&r, dout.ptr, sizeof(double), hipMemcpyDeviceToHost));
       *out = r;
+    });
+  }
+
+  int sbamd_lp_norm(sbamd_ctx* ctx, const sbamd_pcf_set* set, double p, int32_t precision,
+                    void* out_host, int32_t out_dtype)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      need(set, "set");
+      check_out_dtype(out_dtype);
+      if (!(p >= 1.0))
+        fail(SBAMD_ERR_INVALID, "p must be >= 1.");
+      validate_set(*set);
+      if (set->n == 0)
+        return;
+      need(out_host, "out_host");
+      sbamd_set s;
+      make_set(ctx, *set, precision, s);
+      const size_t bytes = static_cast<size_t>(s.n) * (out_dtype == SBAMD_F64 ? 8 : 4);
+      DeviceBuffer dout(bytes);
+      int rc = sbamd_set_lp_norm(&s, p, dout.ptr, out_dtype, ctx->stream);
+      if (rc != SBAMD_OK)
+        fail(rc, g_last_error);
+      SBAMD_HIP(hipStreamSynchronize(ctx->stream));
+      SBAMD_HIP(hipMemcpy(out_host, dout.ptr, bytes, hipMemcpyDeviceToHost));
     });
   }
 

@@ -103,6 +103,19 @@ namespace sbamd
     int32_t out_f64;
   };
 
+  // Kernel parameters of the per-PCF norm kernel.
+  struct NormParams
+  {
+    const int64_t* roff; // sorted record offsets (n + 1)
+    const void* rec;     // records
+    const int64_t* perm; // sorted position -> original index
+    int64_t n;
+    double p;
+    void* out; // n norms in original order
+    int32_t out_f64;
+    int32_t pad;
+  };
+
   struct BwdParams
   {
     FwdParams f;          // forward geometry (out unused)
@@ -234,6 +247,7 @@ namespace sbamd
                       hipStream_t stream);
   void launch_long(const LongParams& prm, int tt_f64, int tv_f64, int op, int kahan, int cu_count,
                    hipStream_t stream);
+  void launch_norm(const NormParams& prm, int tt_f64, int tv_f64, hipStream_t stream);
   void launch_backward(const BwdParams& prm, int op, hipStream_t stream);
   void launch_finalize_grad(const FinalizeParams& prm, hipStream_t stream);
   size_t bwd_lds_bytes(int tv_f64, int R, int C, int cap);

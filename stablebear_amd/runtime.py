@@ -209,6 +209,14 @@ def lp_distance(f: Pcf, g: Pcf, p: float) -> float:
     return float(out.value)
 
 
+def lp_norms(csr: Csr, p: float, precision: int, out: np.ndarray) -> None:
+    """Per-PCF L_p norms of a CSR set into ``out`` (n values, float32 or float64)."""
+    s = csr.struct()
+    out_dtype = L.F64 if out.dtype == np.float64 else L.F32
+    L.check(L.lib().sbamd_lp_norm(context(), ctypes.byref(s), float(p), precision,
+                                  out.ctypes.data if out.size else None, out_dtype))
+
+
 def rectangles(f: Pcf, g: Pcf, a: float, b: float) -> np.ndarray:
     A, B = _one(f), _one(g)
     sa, sb_ = A.struct(), B.struct()
@@ -261,6 +269,11 @@ class DeviceSet:
 
     def update_values(self, v_device_ptr: int, stream: int | None = None) -> None:
         L.check(L.lib().sbamd_set_update_values(self.handle, v_device_ptr, stream))
+
+    def lp_norm(self, p: float, out_ptr: int, out_f64: bool, stream: int | None = None) -> None:
+        """Per-PCF L_p norms into device memory (n values, original order)."""
+        L.check(L.lib().sbamd_set_lp_norm(self.handle, float(p), out_ptr,
+                                          L.F64 if out_f64 else L.F32, stream))
 
     def close(self):
         if getattr(self, "handle", None):

@@ -233,6 +233,9 @@ def test_api_validation_without_device():
         sb.pdist(sb.zeros((2,)), p=0.5)
     with pytest.raises(ValueError, match="p must be >= 1"):
         sb.cdist(sb.zeros((2,)), sb.zeros((3,)), p=0)
+    for p in (0.5, 0, -1):  # test_norms.py:8-23
+        with pytest.raises(ValueError, match="p must be >= 1"):
+            sb.lp_norm(sb.zeros((2,)), p=p)
     with pytest.raises(ValueError, match="1d tensor"):
         sb.pdist(sb.zeros((10, 20)))
     with pytest.raises(ValueError, match="1d tensor"):

@@ -166,6 +166,51 @@ def test_oracle_bitwise_equals_compiled_reference_random(seed):
                                       oracle.pairwise(op, p, kind, off, t, v, use_ref=True))
 
 
+# ---------------------------------------------------------- norms ----------
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("case", CASES["lp_norm"], ids=lambda c: c["src"])
+def test_lp_norm_known_answers(case, dtype):
+    off, t, v = _csr([case["pcf"]])
+    got = float(oracle.lp_norms(float(case["p"]), off, t, v, dtype=dtype)[0])
+    if "abs" in case:
+        assert got == pytest.approx(case["expected"], abs=case["abs"])
+    else:
+        assert got == pytest.approx(case["expected"], rel=case.get("rel", 1e-12), abs=1e-300)
+
+
+@pytest.mark.parametrize("p", [1.0, 2.0, 3.0])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_lp_norm_bitwise_equals_compiled_reference(p, dtype):
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    off, t, v = workloads.c2(200, seed=31)
+    t[5] = t[4]  # a duplicate breakpoint time (zero-length segment)
+    a = oracle.lp_norms(p, off, t, v, dtype=dtype)
+    b = oracle.lp_norms(p, off, t, v, dtype=dtype, use_ref=True)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_lp_norm_l1_matches_exact_rationals():
+    from fractions import Fraction
+    off, t, v = workloads.c1(40, seed=32)
+    got = oracle.lp_norms(1.0, off, t, v)
+    for i in range(len(off) - 1):
+        ts = [Fraction(float(x)) for x in t[off[i]:off[i + 1]]]
+        vs = [Fraction(abs(float(x))) for x in v[off[i]:off[i + 1]]]
+        exact = sum((ts[k + 1] - ts[k]) * vs[k] for k in range(len(ts) - 1))  # last value is 0
+        assert got[i] == pytest.approx(float(exact), rel=1e-14)
+
+
+def test_lp_norm_relates_to_kernel_diagonal():
+    """||f||_2^2 is the L2 kernel's diagonal entry K_ii (polarization check)."""
+    off, t, v = workloads.c2(30, seed=33)
+    n2 = oracle.lp_norms(2.0, off, t, v) ** 2
+    K = oracle.pairwise(oracle.OP_DOT, 1.0, oracle.KIND_KERNEL, off, t, v)
+    i = np.arange(30)
+    np.testing.assert_allclose(n2, K[i * (i + 1) // 2 + i], rtol=1e-12)
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_integrals_match_exact_rationals(seed):
     """L1, squared L2 and dot against exact Fraction sums (one rounding per term)."""
