@@ -114,8 +114,9 @@ static void grad_emit(void* vctx, double l, double r, double fval, double gval,
   }
   if (fi1 != fi0)
   {
-    /* f crossed its breakpoint fi1; partner value is g's pre-step value. */
-    double a = c->fv[fi0], b = c->fv[fi1], s = gval;
+    /* f crossed its breakpoint fi1; the partner value is the mean of g's
+     * values before and after the step (they differ when g crosses too). */
+    double a = c->fv[fi0], b = c->fv[fi1], s = 0.5 * (gval + c->gv[gi1]);
     if (c->op == ORC_OP_DOT)
       c->gtf[fi1] += c->w * (a - b) * s;
     else
@@ -123,8 +124,9 @@ static void grad_emit(void* vctx, double l, double r, double fval, double gval,
   }
   if (gi1 != gi0)
   {
-    /* g crossed its breakpoint gi1; partner value is f's post-step value. */
-    double a = c->gv[gi0], b = c->gv[gi1], s = c->fv[fi1];
+    /* g crossed its breakpoint gi1; the partner value is the mean of f's
+     * values before and after the step. */
+    double a = c->gv[gi0], b = c->gv[gi1], s = 0.5 * (fval + c->fv[fi1]);
     if (c->op == ORC_OP_DOT)
       c->gtg[gi1] += c->w * (a - b) * s;
     else

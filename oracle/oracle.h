@@ -20,8 +20,9 @@
  * Gradients (no reference counterpart): analytic dL/dv and dL/dt of
  * L = sum_pairs G_ij * D_ij (p = 2 distance) or sum_pairs G_ij * K_ij (inner
  * product), following the same walk. Tie convention: when f and g have a
- * breakpoint at the same time, f's breakpoint sees g's left-limit value and
- * g's breakpoint sees f's right-limit value (this makes self-pairs exact).
+ * breakpoint at the same time, each side's breakpoint sees the mean of the
+ * other side's left and right limits (this makes self-pairs exact, and the
+ * two sides add up to the derivative of moving the shared time).
  *
  * Pinning (what checks each function): tests/test_oracle.py.
  */

@@ -39,9 +39,14 @@ def _hipcc() -> str:
     return exe
 
 
+# The backward kernel's atomics come from one lane at a time; the compiler's
+# wave-level atomic combining would wrap each in a loop over the active lanes.
+EXTRA = {"kernels_bwd.hip": ["-mllvm", "-amdgpu-atomic-optimizer-strategy=None"]}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [_hipcc(), *COMMON]
+    cmd = [_hipcc(), *COMMON, *EXTRA.get(src, [])]
     if src.endswith(".cpp"):
         cmd += ["-x", "hip"]
     cmd += ["-c", os.path.join(CSRC, src), "-o", obj]

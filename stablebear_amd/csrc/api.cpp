@@ -963,53 +963,72 @@ extern "C"
       use_device(plan->ctx);
       hipStream_t st = pick_stream(plan->ctx, stream);
       const Set* s = plan->sets[0];
+      if (s->records > INT32_MAX)
+        fail(SBAMD_ERR_INVALID, "backward supports sets of up to 2^31 - 1 records");
       const size_t gb = grad_dtype == SBAMD_F64 ? 8 : 4;
       launch_fill_zero(grad_v_device, static_cast<size_t>(s->points) * gb, st);
       launch_fill_zero(grad_t_device, static_cast<size_t>(s->points) * gb, st);
-      const size_t qbytes = static_cast<size_t>(s->records) * 8;
-      const size_t wbytes = static_cast<size_t>(s->n) * 8;
-      if (plan->bwd_q.bytes < qbytes || plan->bwd_w.bytes < wbytes)
+      const size_t M = static_cast<size_t>(s->records);
+      const size_t N = static_cast<size_t>(s->n);
+      if (plan->bwd_q.bytes < M * 8)
       {
-        plan->bwd_q = DeviceBuffer(qbytes);
-        plan->bwd_a = DeviceBuffer(qbytes);
-        plan->bwd_w = DeviceBuffer(wbytes);
+        // record offsets in the original PCF order
+        std::vector<int64_t> ooff(N + 1, 0);
+        for (size_t q = 0; q < N; ++q)
+          ooff[static_cast<size_t>(s->h_perm[q]) + 1] = s->h_roff[q + 1] - s->h_roff[q];
+        for (size_t i = 0; i < N; ++i)
+          ooff[i + 1] += ooff[i];
+        upload(plan->bwd_ooff, ooff);
+        plan->bwd_key0 = DeviceBuffer(M * 8);
+        plan->bwd_val0 = DeviceBuffer(M * 8);
+        plan->bwd_col0 = DeviceBuffer(M * 4);
+        plan->bwd_src0 = DeviceBuffer(M * 8);
+        for (int b = 0; b < 2; ++b)
+        {
+          plan->bwd_keys[b] = DeviceBuffer(M * 8);
+          plan->bwd_ids[b] = DeviceBuffer(M * 4);
+        }
+        plan->bwd_events = DeviceBuffer(M * sizeof(BwdEvent));
+        plan->bwd_q = DeviceBuffer(M * 8);
+        plan->bwd_a = DeviceBuffer(M * 8);
+        plan->bwd_w = DeviceBuffer(N * 8);
       }
-      launch_fill_zero(plan->bwd_q.ptr, qbytes, st);
-      launch_fill_zero(plan->bwd_a.ptr, qbytes, st);
-      launch_fill_zero(plan->bwd_w.ptr, wbytes, st);
+      launch_fill_zero(plan->bwd_q.ptr, M * 8, st);
+      launch_fill_zero(plan->bwd_a.ptr, M * 8, st);
+      launch_fill_zero(plan->bwd_w.ptr, N * 8, st);
       BwdParams B;
       std::memset(&B, 0, sizeof(B));
-      B.f = fwd_params(*plan, nullptr, out_dtype, nullptr);
-      B.f.work_counter = nullptr;
-      // the accumulator slots add LDS per staged record: a shorter phase
-      // keeps more workgroups resident per CU
-      B.f.cap = std::min(B.f.cap, plan->bwd_cap);
-      while (B.f.cap > 2 && bwd_lds_bytes(plan->tv_f64, B.f.R, B.f.C, B.f.cap) > plan->ctx->lds_per_block)
-        B.f.cap /= 2;
+      B.n = s->n;
+      B.records = s->records;
+      B.op = kop;
+      B.in_f64 = out_dtype == SBAMD_F64 ? 1 : 0;
+      B.w_f64 = plan->tv_f64; // FP64 sets keep FP64 pair weights
+      B.val_f64 = plan->tv_f64;
+      B.grad_f64 = grad_dtype == SBAMD_F64 ? 1 : 0;
+      B.ch = BWD_GROUP;
+      B.roff = s->roff.as<int64_t>();
+      B.rec = s->rec.ptr;
+      B.perm = s->perm.as<int64_t>();
+      B.src = s->src.as<int64_t>();
+      B.ooff = plan->bwd_ooff.as<int64_t>();
+      B.key0 = plan->bwd_key0.as<double>();
+      B.val0 = plan->bwd_val0.as<double>();
+      B.col0 = plan->bwd_col0.as<int32_t>();
+      B.src0 = plan->bwd_src0.as<int64_t>();
+      for (int b = 0; b < 2; ++b)
+      {
+        B.keys[b] = plan->bwd_keys[b].as<double>();
+        B.ids[b] = plan->bwd_ids[b].as<int32_t>();
+      }
+      B.events = plan->bwd_events.as<BwdEvent>();
       B.grad_out = grad_out_device;
       B.fwd_out = fwd_out_device;
       B.q = plan->bwd_q.as<double>();
       B.a = plan->bwd_a.as<double>();
       B.wsum = plan->bwd_w.as<double>();
-      B.in_f64 = out_dtype == SBAMD_F64 ? 1 : 0;
-      B.val_f64 = plan->tv_f64;
-      launch_backward(B, kop, st);
-      FinalizeParams F;
-      std::memset(&F, 0, sizeof(F));
-      F.roff = s->roff.as<int64_t>();
-      F.rec = s->rec.ptr;
-      F.src = s->src.as<int64_t>();
-      F.q = B.q;
-      F.a = B.a;
-      F.wsum = B.wsum;
-      F.grad_v = grad_v_device;
-      F.grad_t = grad_t_device;
-      F.n = s->n;
-      F.records = s->records;
-      F.val_f64 = plan->tv_f64;
-      F.grad_f64 = grad_dtype == SBAMD_F64 ? 1 : 0;
-      F.op = kop;
-      launch_finalize_grad(F, st);
+      B.grad_v = grad_v_device;
+      B.grad_t = grad_t_device;
+      launch_backward(B, st);
     });
   }
 

@@ -1,321 +1,542 @@
 // Backward of L = sum_e G_e * X_e over the pair outputs X of a pdist (p = 2)
 // or l2_kernel plan. No reference counterpart (the reference has no autograd).
 //
-// For a PCF f with records r (value v_r on [t_start(r), t_end(r))) and its
-// partners g_j with pair weights w_j (G / (2 D) for the L2 distance
-// D = sqrt(int (f - g)^2), G for the inner product K = int f g):
+// For a PCF f = f_i with records r (value v_r on [t_start(r), t_end(r))) and
+// its partners g_j with pair weights w_ij (G / (2 D) for the L2 distance
+// D = sqrt(int (f - g)^2), G for the inner product K = int f g, 2 G for the
+// kernel's diagonal entry int f^2):
 //   dL/dv_r  (L2)  = 2 (v_r len_r W - Q_r)        (dot) = Q_r
 //   dL/dt_r  (L2)  = (v_{r-1} - v_r) ((v_{r-1} + v_r) W - 2 A_{r-1})
 //            (dot) = (v_{r-1} - v_r) A_{r-1}
-// with W = sum_j w_j, Q_r = sum_j w_j int_{record r} g_j, and A_r = sum_j w_j
-// g_j(t_end(r)) (the partner value where record r ends). The tile kernel
-// walks every pair once (the forward walk) accumulating Q and A per record:
-// w times the integral of the partner over the current record in a
-// register, added to the record's LDS slot together with w times the partner
-// value when the walk leaves the record; the slots go to global memory once
-// per phase. A small finalize kernel applies the formulas above.
+// with W = sum_j w_ij, Q_r = integral of H_i over record r, A_r = H_i at
+// t_end(r), and H_i = sum_j w_ij g_j.
 //
-// Ties: where f and g end a record at the same time, the side whose record
-// the walk already holds as the earlier-ending one leaves first and sees the
-// partner's left-limit value; the other side leaves second and sees the
-// first side's new, right-limit value. The derivative does not exist there;
-// for a self pair (kernel diagonal) the two halves add up to the exact
-// derivative whichever side leaves first.
+// H_i is piecewise constant and changes only where a partner has a
+// breakpoint, by w_ij times that partner's jump. PCFs are taken in groups of
+// 64 consecutive ones; each group's record starts, merged into one
+// time-ordered event list, serve twice. A wavefront handles a band of 64
+// rows (one lane per row) against a chunk of 64 partners by merging two
+// wave-uniform event lists staged in LDS: the chunk's list, where every
+// event moves all lanes' H_i (w_ij from LDS), and the band's own list, where
+// an event is one lane's breakpoint: that lane emits the Q and A of the
+// record that ends there. Control flow stays uniform, nothing is shared
+// between lanes, and the cost is the two lists' length; the chunks of a band
+// add their Q and A with global atomics.
+//
+// All arrays here are in the original PCF order ("position" p of record k of
+// PCF i is ooff[i] + k); a layout kernel copies them out of the sorted
+// device set, and the group event lists come from a merge sort by time.
+//
+// Ties: where f and a partner have a breakpoint at the same time, A takes the
+// mean of H_i's left and right limits there (the mean of the partner's values
+// on both sides). For a self pair (kernel diagonal) that is the exact
+// derivative; for two PCFs jumping together the two one-sided results add up
+// to the derivative of moving the shared time.
 #include "walk_common.hpp"
 
 namespace sbamd
 {
-  template <typename TV, int OP>
-  __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void bwd_tile_kernel(BwdParams B)
+  namespace
   {
-    extern __shared__ __align__(16) unsigned char smem[];
-    using TT = double;
-    using L = Lim<TT>;
-    using RecT = Rec<TT, TV>;
-    const FwdParams& P = B.f;
-
-    // 16 bytes of accumulators (Q, A) per staged record, then the W per slot
-    const LdsLayout lay = lds_layout(sizeof(TT), P.R, P.C, P.cap, 16);
-    const TileLds<TT, TV> S(smem, lay);
-    static_assert(sizeof(RecT) == 16, "accumulator slots mirror 16-byte records");
-    constexpr uint32_t REC = sizeof(RecT);
-    // Q and A are two arrays of one FP64 per staged record (8-byte stride,
-    // so the lanes' atomics of one instruction spread over all LDS banks);
-    // record at LDS byte address x has its slots at QOFF / AOFF + x / 2.
-    const uint32_t QOFF = static_cast<uint32_t>(lay.extra_off);
-    const uint32_t AOFF = QOFF + static_cast<uint32_t>(lay.rec_end / 2);
-    double* const s_w = reinterpret_cast<double*>(smem + lay.total);
-
-    const int tid = threadIdx.x;
-    const int nthreads = blockDim.x;
-    const int NS = P.R + P.C;
-    const uint32_t SR = static_cast<uint32_t>(slot_recs(P.cap));
-
-    for (int64_t ti = blockIdx.x; ti < P.ntiles; ti += gridDim.x)
+    __device__ __forceinline__ double load_out(const void* p, int64_t idx, int f64)
     {
-      const Tile tl = P.tiles[ti];
-      const Problem pr = P.probs[tl.prob];
+      return f64 ? static_cast<const double*>(p)[idx]
+                 : static_cast<double>(static_cast<const float*>(p)[idx]);
+    }
 
-      tile_begin(S, P, tl, pr, tid);
-      if (tid < NS)
-        s_w[tid] = 0.0;
-      __syncthreads();
-
-      // Lanes take the tile's pairs along diagonals, (r, (r + k) mod C) with
-      // r = tid mod R and k = tid / R, so a wavefront holds each row and each
-      // column only a few times: the lanes of one PCF that leave the same
-      // record in one instruction add to the same LDS word, and the
-      // hardware serialises those adds.
-      const int lr = tid % P.R;
-      const int lc = (lr + tid / P.R) % P.C;
-      bool valid = tid < P.R * P.C && lr < tl.nr && lc < tl.nc;
-      if (valid && (tl.flags & TILE_DIAG))
-        valid = P.include_diag ? (lr >= lc) : (lr > lc);
-
-      double w = 0.0;
-      if (valid)
+    // w_ij of the loss for rows i and partner j (original indices).
+    template <int OP>
+    __device__ __forceinline__ double pair_weight(const BwdParams& B, int64_t i, int64_t j)
+    {
+      if (OP == OPK_DOT)
       {
-        const int64_t i = P.r_perm[pr.row_base + tl.r0 + lr];
-        const int64_t j = P.c_perm[pr.col_base + tl.c0 + lc];
-        const int64_t idx = (P.out_kind == SBAMD_OUT_DISTMAT ? distmat_index(i, j) : symmat_index(i, j)) + pr.out_base;
-        const double G = B.in_f64 ? static_cast<const double*>(B.grad_out)[idx]
-                                  : static_cast<double>(static_cast<const float*>(B.grad_out)[idx]);
-        if constexpr (OP == OPK_DOT)
-        {
-          w = G;
-        }
+        const double G = load_out(B.grad_out, symmat_index(i, j), B.in_f64);
+        return i == j ? 2.0 * G : G;
+      }
+      if (i == j)
+        return 0.0;
+      const int64_t idx = distmat_index(i, j);
+      const double D = load_out(B.fwd_out, idx, B.in_f64);
+      return D > 0.0 ? load_out(B.grad_out, idx, B.in_f64) / (2.0 * D) : 0.0;
+    }
+
+    // Number of keys in [lo, hi) below `key` (strict) or not above it.
+    __device__ __forceinline__ int64_t count_before(const double* keys, int64_t lo, int64_t hi,
+                                                    double key, bool inclusive)
+    {
+      while (lo < hi)
+      {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        const double k = keys[mid];
+        if (inclusive ? k <= key : k < key)
+          lo = mid + 1;
         else
-        {
-          const double D = B.in_f64 ? static_cast<const double*>(B.fwd_out)[idx]
-                                    : static_cast<double>(static_cast<const float*>(B.fwd_out)[idx]);
-          w = D > 0.0 ? G / (2.0 * D) : 0.0;
-        }
-        if (w == 0.0)
-          valid = false;
+          hi = mid;
       }
-      const int slot_f = valid ? lr : 0;
-      const int slot_g = valid ? P.R + lc : P.R;
-      if (valid)
+      return lo;
+    }
+
+    // Wave-uniform loads through the scalar cache: the pointer is cast to the
+    // constant address space, which the compiler reads with s_load.
+    template <typename T>
+    __device__ __forceinline__ T uniform_load(const T* p, int64_t i)
+    {
+      return ((const __attribute__((address_space(4))) T*)(p))[i];
+    }
+
+    __device__ __forceinline__ BwdEvent uniform_event(const BwdEvent* p, int64_t i)
+    {
+      const __attribute__((address_space(4))) BwdEvent* c =
+          (const __attribute__((address_space(4))) BwdEvent*)(p);
+      BwdEvent e;
+      e.t = c[i].t;
+      e.dt = c[i].dt;
+      e.d = c[i].d;
+      e.col = c[i].col;
+      e.pos = c[i].pos;
+      return e;
+    }
+
+    __device__ __forceinline__ double uniform_time(const BwdEvent* p, int32_t i)
+    {
+      return ((const __attribute__((address_space(4))) BwdEvent*)(p))[i].t;
+    }
+
+    // First index of a time-ordered event list of n events whose time is >= t.
+    __device__ __forceinline__ int32_t lower_bound_time(const BwdEvent* p, int32_t n, double t)
+    {
+      int32_t lo = 0, hi = n;
+      while (lo < hi)
       {
-        atomicAdd(&s_w[slot_f], w);
-        atomicAdd(&s_w[slot_g], w);
+        const int32_t mid = (lo + hi) >> 1;
+        if (uniform_time(p, mid) < t)
+          lo = mid + 1;
+        else
+          hi = mid;
       }
+      return lo;
+    }
 
-      // Walk state, values pre-multiplied by the pair weight w: lo is the
-      // side whose current record ends first (at lo_t), hi the other side.
-      TT t = TT(0);
-      TT lo_t = L::inf(), hi_t = L::inf();
-      double lo_w = 0.0, hi_w = 0.0;     // w * current value of each side
-      double lo_acc = 0.0, hi_acc = 0.0; // w * integral of the partner over the current record
-      int lo_s = slot_f, hi_s = slot_g;
-      int64_t lo_k = 1, hi_k = 1; // record index of each side's next record
-
-      for (int phase = 0;; ++phase)
-      {
-        phase_stage<TT, TV, 1>(S, P, phase, tid, nthreads);
-        for (int k = tid; k < lay.rec_end / 16; k += nthreads)
-          *reinterpret_cast<double2*>(smem + QOFF + 16 * k) = make_double2(0.0, 0.0);
-        __syncthreads();
-        const TT Tb = L::from_bits(S.tb[phase & 1]);
-        phase_advance_cursor<TT, TV, 1>(S, P, Tb, tid);
-
-        if (valid)
-        {
-          if (phase == 0)
-          {
-            // record 0 sits at local index 1 (index 0 is the lead slot)
-            const RecT rf = S.rec(S.lbase[slot_f] + REC);
-            const RecT rg = S.rec(S.lbase[slot_g] + REC);
-            const bool k = rf.t <= rg.t;
-            lo_t = k ? rf.t : rg.t;
-            lo_w = w * static_cast<double>(k ? rf.v : rg.v);
-            hi_t = k ? rg.t : rf.t;
-            hi_w = w * static_cast<double>(k ? rg.v : rf.v);
-            lo_s = k ? slot_f : slot_g;
-            hi_s = k ? slot_g : slot_f;
-          }
-          // window: record k at local k - cur + 1
-          uint32_t lo_p = S.lbase[lo_s] + static_cast<uint32_t>(lo_k - S.cur[lo_s] + 1) * REC;
-          uint32_t hi_p = S.lbase[hi_s] + static_cast<uint32_t>(hi_k - S.cur[hi_s] + 1) * REC;
-          while (lo_t < Tb)
-          {
-            const double len = lo_t - t;
-            lo_acc = fma(hi_w, len, lo_acc);
-            hi_acc = fma(lo_w, len, hi_acc);
-            t = lo_t;
-            // lo leaves its current record (the one before lo_p)
-            const uint32_t half = (lo_p - REC) >> 1;
-            atomicAdd(reinterpret_cast<double*>(smem + QOFF + half), lo_acc);
-            atomicAdd(reinterpret_cast<double*>(smem + AOFF + half), hi_w);
-            const RecT r = S.rec(lo_p);
-            const double rw = w * static_cast<double>(r.v);
-            const uint32_t np = lo_p + REC;
-            const bool k = r.t <= hi_t;
-            const TT nlt = tmin(r.t, hi_t);
-            const double nlw = k ? rw : hi_w;
-            const uint32_t nlp = k ? np : hi_p;
-            const double nla = k ? 0.0 : hi_acc;
-            hi_t = tmax(r.t, hi_t);
-            hi_w = k ? hi_w : rw;
-            hi_p = k ? hi_p : np;
-            hi_acc = k ? hi_acc : 0.0;
-            lo_t = nlt;
-            lo_w = nlw;
-            lo_p = nlp;
-            lo_acc = nla;
-          }
-          // a next-record pointer is at local 1 .. cap + 1 (slot_recs >= cap + 1),
-          // so local - 1 identifies the slot even at the end of a full window
-          const uint32_t lo_i = lo_p / REC, hi_i = hi_p / REC;
-          lo_s = static_cast<int>((lo_i - 1) / SR);
-          hi_s = static_cast<int>((hi_i - 1) / SR);
-          lo_k = S.cur[lo_s] - 1 + (lo_i - S.lbase[lo_s] / REC);
-          hi_k = S.cur[hi_s] - 1 + (hi_i - S.lbase[hi_s] / REC);
-        }
-        __syncthreads();
-
-        // this phase's accumulators: records cur - 1 .. next - 1 of each slot
-        for (int idx = tid; idx < NS * (P.cap + 1); idx += nthreads)
-        {
-          const int s = idx / (P.cap + 1);
-          const int j = idx - s * (P.cap + 1);
-          const int64_t k = S.cur[s] - 1 + j;
-          if (k >= 0 && k < S.next[s])
-          {
-            const uint32_t half = (S.lbase[s] + static_cast<uint32_t>(j) * REC) >> 1;
-            const double q = *reinterpret_cast<const double*>(smem + QOFF + half);
-            const double a = *reinterpret_cast<const double*>(smem + AOFF + half);
-            const int64_t g = S.base[s] + k;
-            if (q != 0.0)
-              atomicAdd(&B.q[g], q);
-            if (a != 0.0)
-              atomicAdd(&B.a[g], a);
-          }
-        }
-        __syncthreads();
-        if (!(Tb < L::inf()))
-          break;
-      }
-
-      if (valid)
-      {
-        // final rectangle [t, MAX), then the last records' integrals
-        const double len = L::max() - t;
-        if (hi_w != 0.0)
-          lo_acc = fma(hi_w, len, lo_acc);
-        if (lo_w != 0.0)
-          hi_acc = fma(lo_w, len, hi_acc);
-        atomicAdd(&B.q[S.base[lo_s] + lo_k - 1], lo_acc);
-        atomicAdd(&B.q[S.base[hi_s] + hi_k - 1], hi_acc);
-      }
-      if (tid < NS)
-      {
-        const bool isrow = tid < P.R;
-        const int kk = isrow ? tid : tid - P.R;
-        if (kk < (isrow ? tl.nr : tl.nc) && s_w[tid] != 0.0)
-          atomicAdd(&B.wsum[isrow ? pr.row_base + tl.r0 + kk : pr.col_base + tl.c0 + kk], s_w[tid]);
-      }
-      __syncthreads();
+    // Records at positions [a, b) of one PCF that start before t.
+    __device__ __forceinline__ int64_t starts_below(const double* key0, int64_t a, int64_t b, double t)
+    {
+      return count_before(key0, a, b, t, false) - a;
     }
   }
 
-  // One thread per record: the formulas in the file comment.
-  __global__ void finalize_grad_kernel(FinalizeParams F)
+  constexpr int BWD_WAVES = 4; // wavefronts per backward workgroup (time slices of one task)
+  constexpr int BWD_BLK = 32;  // events per staged LDS block of one list
+
+  // LDS bytes of the pair weights and of the whole workgroup: weights, and
+  // per wave the partners' start values (64 FP64), two staged blocks and 64
+  // pending emissions (Q, A, record).
+  __host__ __device__ constexpr size_t bwd_lds_weights(size_t wbytes)
+  {
+    return static_cast<size_t>(BWD_GROUP) * 64 * wbytes;
+  }
+  __host__ __device__ constexpr size_t bwd_lds_bytes(size_t wbytes)
+  {
+    return bwd_lds_weights(wbytes) + BWD_WAVES * (64 * sizeof(double) + 2 * BWD_BLK * sizeof(BwdEvent) +
+                                                  2 * 64 * sizeof(double) + 64 * sizeof(int32_t));
+  }
+
+  // Thread per sorted record: copies start time, value, owner and source
+  // point into the original-order position layout and seeds the sort keys.
+  __global__ void bwd_layout_kernel(BwdParams B)
   {
     const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= F.records)
+    if (r >= B.records)
       return;
-    int64_t lo = 0, hi = F.n; // PCF q with roff[q] <= r < roff[q + 1]
+    int64_t lo = 0, hi = B.n; // sorted PCF s with roff[s] <= r < roff[s + 1]
     while (hi - lo > 1)
     {
       const int64_t mid = (lo + hi) >> 1;
-      if (F.roff[mid] <= r)
+      if (B.roff[mid] <= r)
         lo = mid;
       else
         hi = mid;
     }
-    const int64_t q = lo;
-    const int64_t first = F.roff[q];
-    // 16-byte records: FP64 end time, then the value (FP64 or FP32)
-    const unsigned char* rec = static_cast<const unsigned char*>(F.rec);
-    auto tend = [&](int64_t k) { return *reinterpret_cast<const double*>(rec + 16 * k); };
-    auto val = [&](int64_t k) {
-      return F.val_f64 ? *reinterpret_cast<const double*>(rec + 16 * k + 8)
-                       : static_cast<double>(*reinterpret_cast<const float*>(rec + 16 * k + 8));
-    };
-    const double v = val(r);
-    const double W = F.wsum[q];
-    double gv, gt = 0.0;
-    if (F.op == OPK_DOT)
+    const int64_t k = r - B.roff[lo];
+    const int64_t i = B.perm[lo];
+    const int64_t p = B.ooff[i] + k;
+    const unsigned char* rec = static_cast<const unsigned char*>(B.rec);
+    const double start = k ? *reinterpret_cast<const double*>(rec + 16 * (r - 1)) : 0.0;
+    const double v = B.val_f64 ? *reinterpret_cast<const double*>(rec + 16 * r + 8)
+                               : static_cast<double>(*reinterpret_cast<const float*>(rec + 16 * r + 8));
+    B.key0[p] = start;
+    B.val0[p] = v;
+    B.col0[p] = static_cast<int32_t>(i);
+    B.src0[p] = B.src[r];
+    B.keys[0][p] = start;
+    B.ids[0][p] = static_cast<int32_t>(p);
+  }
+
+  // One level of the per-chunk merge sort: runs of `width` consecutive PCFs
+  // (each PCF's records are already in time order) merge pairwise; equal
+  // times keep the left run first, so the order is (time, PCF, record).
+  __global__ void bwd_merge_kernel(BwdParams B, int32_t width, int32_t from)
+  {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= B.records)
+      return;
+    const double* kin = B.keys[from];
+    const int32_t* iin = B.ids[from];
+    const int64_t i = B.col0[e];
+    const int64_t cb = (i / B.ch) * B.ch;
+    const int64_t ce = min(cb + B.ch, B.n);
+    const int64_t run = (i - cb) / width;
+    const int64_t sib = run ^ 1;
+    const int64_t own0 = cb + run * width;
+    const int64_t sib0 = cb + sib * width;
+    const double key = kin[e];
+    int64_t rank = 0;
+    if (sib0 < ce)
     {
-      gv = F.q[r];
-      if (r > first)
-        gt = (val(r - 1) - v) * F.a[r - 1];
+      const int64_t lo = B.ooff[sib0], hi = B.ooff[min(sib0 + width, ce)];
+      rank = count_before(kin, lo, hi, key, (run & 1) != 0) - lo;
     }
-    else
+    const int64_t pos = B.ooff[cb + (run & ~int64_t(1)) * width] + (e - B.ooff[own0]) + rank;
+    B.keys[from ^ 1][pos] = key;
+    B.ids[from ^ 1][pos] = iin[e];
+  }
+
+  // Thread per position of the sorted chunk lists: the event record.
+  __global__ void bwd_events_kernel(BwdParams B, int32_t from)
+  {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= B.records)
+      return;
+    const int64_t cb = (static_cast<int64_t>(B.col0[p]) / B.ch) * B.ch;
+    const double t = B.keys[from][p];
+    const double tp = p > B.ooff[cb] ? B.keys[from][p - 1] : 0.0;
+    const int64_t id = B.ids[from][p];
+    const int64_t c = B.col0[id];
+    const bool first = id == B.ooff[c];
+    BwdEvent ev;
+    ev.t = t;
+    ev.dt = t - tp;
+    ev.d = B.val0[id] - (first ? 0.0 : B.val0[id - 1]);
+    ev.col = static_cast<int32_t>(c - cb) | (first ? BWD_FIRST : 0);
+    ev.pos = static_cast<int32_t>(id);
+    B.events[p] = ev;
+  }
+
+  // One workgroup per (band of 64 rows, chunk of 64 partners), four
+  // wavefronts splitting the task's time axis: wave k takes the events in
+  // [T_k, T_{k+1}), with T_k the time of the chunk's event at k/4 of its
+  // list. A wave starts from H_i just before T_k (from the partners' values
+  // there) and from the row record open there; the part of that record's
+  // integral before T_k was added by wave k - 1. Each wave stages both lists
+  // in LDS blocks of BWD_BLK events and reads them at wave-uniform
+  // addresses. Workgroups are dealt to the 8 XCDs round-robin; all bands of a
+  // chunk run on one XCD so the chunk's list stays in that XCD's L2.
+  template <typename WT, int OP>
+  __global__ __launch_bounds__(64 * BWD_WAVES) void bwd_stream_kernel(BwdParams B)
+  {
+    extern __shared__ __align__(16) unsigned char smem[];
+    WT* const s_w = reinterpret_cast<WT*>(smem); // s_w[jj * 64 + ii] = w(row ii, partner jj)
+    double* const s_g = reinterpret_cast<double*>(smem + bwd_lds_weights(sizeof(WT)));
+    BwdEvent* const s_ev = reinterpret_cast<BwdEvent*>(s_g + BWD_WAVES * 64);
+    double* const s_em = reinterpret_cast<double*>(s_ev + BWD_WAVES * 2 * BWD_BLK);
+    int32_t* const s_ip = reinterpret_cast<int32_t*>(s_em + BWD_WAVES * 2 * 64);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    // the wave index, as a wave-uniform value, so that everything derived
+    // from it (slice bounds, list indices) stays in scalar registers
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int64_t ng = (B.n + BWD_GROUP - 1) / BWD_GROUP;
+    const int64_t x = blockIdx.x;
+    const int64_t slot = x >> 3;
+    const int64_t c = (x & 7) + 8 * (slot / ng);
+    if (c >= ng)
+      return;
+    const int64_t band = slot % ng;
+    const int64_t i = band * BWD_GROUP + lane;
+    const bool act = i < B.n;
+    const int64_t j0 = c * BWD_GROUP;
+    const int ncol = static_cast<int>(min(static_cast<int64_t>(BWD_GROUP), B.n - j0));
+    const int nrow = static_cast<int>(min(static_cast<int64_t>(BWD_GROUP), B.n - band * BWD_GROUP));
+
+    // pair weights, partner-major: the lanes read one row of 64 consecutive
+    // words per event
+    for (int k = tid; k < BWD_GROUP * 64; k += 64 * BWD_WAVES)
     {
-      // the last record ends where the domain does, at DBL_MAX
-      const double end = tend(r) < __builtin_inf() ? tend(r) : DBL_MAX;
-      const double len = end - (r > first ? tend(r - 1) : 0.0);
-      gv = 2.0 * ((v != 0.0 ? v * len * W : 0.0) - F.q[r]);
-      if (r > first)
+      const int jj = k >> 6, ii = k & 63;
+      const double w = jj < ncol && ii < nrow ? pair_weight<OP>(B, band * BWD_GROUP + ii, j0 + jj) : 0.0;
+      s_w[k] = static_cast<WT>(w);
+    }
+    __syncthreads();
+    if (wv == 0 && act)
+    {
+      double ws = 0.0;
+      for (int jj = 0; jj < ncol; ++jj)
+        ws += static_cast<double>(s_w[jj * 64 + lane]);
+      if (ws != 0.0)
+        atomicAdd(&B.wsum[i], ws);
+    }
+
+    // this wave's slice of both lists
+    const int64_t r0 = uniform_load(B.ooff, band * BWD_GROUP);
+    const int32_t nr = static_cast<int32_t>(uniform_load(B.ooff, band * BWD_GROUP + nrow) - r0);
+    const int64_t c0 = uniform_load(B.ooff, j0);
+    const int32_t ncv = static_cast<int32_t>(uniform_load(B.ooff, j0 + ncol) - c0);
+    const BwdEvent* const rl = B.events + r0;
+    const BwdEvent* const cl = B.events + c0;
+    const bool last = wv == BWD_WAVES - 1;
+    double tlo = 0.0, thi = __builtin_inf();
+    int32_t cb = 0, ce = ncv, rb = 0, re = nr;
+    if (wv > 0)
+    {
+      tlo = uniform_time(cl, static_cast<int32_t>((static_cast<int64_t>(wv) * ncv) / BWD_WAVES));
+      cb = lower_bound_time(cl, ncv, tlo);
+      rb = lower_bound_time(rl, nr, tlo);
+    }
+    if (!last)
+    {
+      thi = uniform_time(cl, static_cast<int32_t>((static_cast<int64_t>(wv + 1) * ncv) / BWD_WAVES));
+      ce = lower_bound_time(cl, ncv, thi);
+      re = lower_bound_time(rl, nr, thi);
+    }
+
+    // H_i just before tlo, from the partners' values there (lane = partner)
+    double H = 0.0;
+    if (wv > 0)
+    {
+      double gv = 0.0;
+      if (lane < ncol)
       {
-        const double vp = val(r - 1);
-        gt = (vp - v) * ((vp + v) * W - 2.0 * F.a[r - 1]);
+        const int64_t a = B.ooff[j0 + lane];
+        const int64_t k = starts_below(B.key0, a, B.ooff[j0 + lane + 1], tlo);
+        if (k > 0)
+          gv = B.val0[a + k - 1];
       }
+      s_g[wv * 64 + lane] = gv;
+      for (int jj = 0; jj < ncol; ++jj)
+        H = fma(static_cast<double>(s_w[jj * 64 + lane]), s_g[wv * 64 + jj], H);
     }
-    const int64_t pt = F.src[r];
+    // the row record open just before tlo
+    int64_t open = 0;
+    if (act)
+    {
+      const int64_t a = B.ooff[i];
+      const int64_t k = wv > 0 ? starts_below(B.key0, a, B.ooff[i + 1], tlo) : 0;
+      open = a + (k > 0 ? k - 1 : 0);
+    }
+    // past the chunk's last event H_i is sum_j w_ij (last value of g_j),
+    // taken exactly there rather than as the running sum of the jumps
+    double Hend = 0.0;
+    if (last)
+      for (int jj = 0; jj < ncol; ++jj)
+        Hend = fma(static_cast<double>(s_w[jj * 64 + lane]),
+                   B.val0[uniform_load(B.ooff, j0 + jj + 1) - 1], Hend);
+
+    // Pending emissions (record, Q, A) go out 64 at a time, one lane per
+    // record. A flush follows a staged block's store, so no load of the
+    // event lists ever waits for these atomics (vector memory operations
+    // complete in order).
+    double* const em_q = s_em + wv * 2 * 64;
+    double* const em_a = em_q + 64;
+    int32_t* const em_p = s_ip + wv * 64;
+    int32_t nem = 0;
+    auto flush = [&]() {
+      if (lane < nem)
+      {
+        atomicAdd(&B.q[em_p[lane]], em_q[lane]);
+        atomicAdd(&B.a[em_p[lane]], em_a[lane]);
+      }
+      nem = 0;
+    };
+    BwdEvent* const bufC = s_ev + wv * 2 * BWD_BLK;
+    BwdEvent* const bufR = bufC + BWD_BLK;
+    // Every lane loads and stores (index clamped; lanes 32..63 repeat lanes
+    // 0..31, slots past the list end are never read), so no loaded register
+    // stays pending past the store.
+    auto stage = [&](BwdEvent* buf, const BwdEvent* src, int32_t s0, int32_t end) {
+      if (s0 < end)
+      {
+        const int32_t k = min(s0 + (lane & (BWD_BLK - 1)), end - 1);
+        const double4 v4 = *reinterpret_cast<const double4*>(src + k);
+        *reinterpret_cast<double4*>(buf + (lane & (BWD_BLK - 1))) = v4;
+      }
+      flush();
+    };
+
+    int32_t cs = cb, rs = rb; // first event of the staged blocks
+    stage(bufC, cl, cb, ce);
+    stage(bufR, rl, rb, re);
+    double acc = 0.0;  // integral of H_i from the open record's start (or tlo) to tprev
+    double tprev = tlo; // time of the chunk's last event taken
+    int32_t ic = cb, ir = rb;
+    bool hc = ic < ce;
+    BwdEvent C = bufC[0];
+    double Rt = ir < re ? bufR[0].t : __builtin_inf();
+    for (;;)
+    {
+      // the chunk's events up to the band's next breakpoint (chunk first on
+      // equal times, so H is the right limit at a breakpoint)
+      while (hc && C.t <= Rt)
+      {
+        const double w = static_cast<double>(s_w[(C.col & (BWD_GROUP - 1)) * 64 + lane]);
+        acc = fma(H, C.t - tprev, acc);
+        H = fma(w, C.d, H);
+        tprev = C.t;
+        hc = ++ic < ce;
+        if (hc)
+        {
+          if (ic == cs + BWD_BLK)
+          {
+            cs = ic;
+            stage(bufC, cl, cs, ce);
+          }
+          C = bufC[ic - cs];
+        }
+      }
+      if (last && !hc)
+        H = Hend;
+      if (ir >= re)
+        break;
+      const BwdEvent R = bufR[ir - rs];
+      if (!(R.col & BWD_FIRST))
+      {
+        // record R.pos - 1 of lane R.col's PCF ends at R.t
+        const double d = R.t - tprev;
+        double J = 0.0;
+        if (ic > cb && __double_as_longlong(R.t) == __double_as_longlong(tprev))
+        {
+          // partners jump at R.t too: A is the mean of H's two limits
+          for (int32_t e = ic - 1; e >= cb; --e)
+          {
+            const BwdEvent T = e >= cs ? bufC[e - cs] : uniform_event(cl, e);
+            if (__double_as_longlong(T.t) != __double_as_longlong(tprev))
+              break;
+            J = fma(static_cast<double>(s_w[(T.col & (BWD_GROUP - 1)) * 64 + lane]), T.d, J);
+          }
+        }
+        if (lane == R.col)
+        {
+          em_q[nem] = fma(H, d, acc);
+          em_a[nem] = fma(-0.5, J, H);
+          em_p[nem] = R.pos - 1;
+          acc = -H * d;
+          open = R.pos;
+        }
+        if (++nem == 64)
+          flush();
+      }
+      if (++ir < re)
+      {
+        if (ir == rs + BWD_BLK)
+        {
+          rs = ir;
+          stage(bufR, rl, rs, re);
+        }
+        Rt = bufR[ir - rs].t;
+      }
+      else
+        Rt = __builtin_inf();
+    }
+    flush();
+    if (act)
+    {
+      // the open record's integral up to thi; the last record runs to the
+      // end of the domain, DBL_MAX
+      double qv;
+      if (!last)
+        qv = fma(H, thi - tprev, acc);
+      else
+        qv = H != 0.0 ? fma(H, DBL_MAX - tprev, acc) : acc;
+      atomicAdd(&B.q[open], qv);
+    }
+  }
+
+  // Thread per position: the formulas in the file comment.
+  __global__ void finalize_grad_kernel(BwdParams B)
+  {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= B.records)
+      return;
+    const int64_t pt = B.src0[p];
     if (pt < 0) // the zero record standing in for an empty PCF
       return;
-    if (F.grad_f64)
+    const int64_t i = B.col0[p];
+    const int64_t first = B.ooff[i];
+    const bool last = p + 1 == B.ooff[i + 1];
+    const double v = B.val0[p];
+    double gv, gt = 0.0;
+    if (B.op == OPK_DOT)
     {
-      static_cast<double*>(F.grad_v)[pt] = gv;
-      static_cast<double*>(F.grad_t)[pt] = gt;
+      gv = B.q[p];
+      if (p > first)
+        gt = (B.val0[p - 1] - v) * B.a[p - 1];
     }
     else
     {
-      static_cast<float*>(F.grad_v)[pt] = static_cast<float>(gv);
-      static_cast<float*>(F.grad_t)[pt] = static_cast<float>(gt);
+      const double W = B.wsum[i];
+      // the last record ends where the domain does, at DBL_MAX
+      const double len = (last ? DBL_MAX : B.key0[p + 1]) - B.key0[p];
+      gv = 2.0 * ((v != 0.0 ? v * len * W : 0.0) - B.q[p]);
+      if (p > first)
+      {
+        const double vp = B.val0[p - 1];
+        gt = (vp - v) * ((vp + v) * W - 2.0 * B.a[p - 1]);
+      }
+    }
+    if (B.grad_f64)
+    {
+      static_cast<double*>(B.grad_v)[pt] = gv;
+      static_cast<double*>(B.grad_t)[pt] = gt;
+    }
+    else
+    {
+      static_cast<float*>(B.grad_v)[pt] = static_cast<float>(gv);
+      static_cast<float*>(B.grad_t)[pt] = static_cast<float>(gt);
     }
   }
 
-  size_t bwd_lds_bytes(int tv_f64, int R, int C, int cap)
+  void launch_backward(const BwdParams& prm, hipStream_t stream)
   {
-    (void)tv_f64;
-    return static_cast<size_t>(lds_layout(8, R, C, cap, 16).total) + 8 * static_cast<size_t>(R + C);
-  }
-
-  void launch_backward(const BwdParams& prm, int op, hipStream_t stream)
-  {
-    const FwdParams& P = prm.f;
-    const unsigned grid = static_cast<unsigned>(P.ntiles < 0x7fffffff ? P.ntiles : 0x7fffffff);
-    if (grid == 0)
+    if (prm.records <= 0 || prm.n <= 0)
       return;
-    const size_t lds = bwd_lds_bytes(prm.val_f64, P.R, P.C, P.cap);
-    const dim3 block(tile_block_threads(P.R, P.C));
-    if (prm.val_f64)
+    if (prm.ch != BWD_GROUP)
+      fail(SBAMD_ERR_RUNTIME, "backward: group size must be 64");
+    const unsigned rgrid = static_cast<unsigned>((prm.records + 255) / 256);
+    hipLaunchKernelGGL(bwd_layout_kernel, dim3(rgrid), dim3(256), 0, stream, prm);
+    SBAMD_HIP(hipGetLastError());
+    int from = 0;
+    for (int32_t width = 1; width < prm.ch; width *= 2)
     {
-      if (op == OPK_DOT)
-        hipLaunchKernelGGL((bwd_tile_kernel<double, OPK_DOT>), dim3(grid), block, lds, stream, prm);
+      hipLaunchKernelGGL(bwd_merge_kernel, dim3(rgrid), dim3(256), 0, stream, prm, width, from);
+      SBAMD_HIP(hipGetLastError());
+      from ^= 1;
+    }
+    hipLaunchKernelGGL(bwd_events_kernel, dim3(rgrid), dim3(256), 0, stream, prm, from);
+    SBAMD_HIP(hipGetLastError());
+
+    // (band, chunk) workgroups: every band of 64 rows against every chunk of
+    // 64 partners, chunk-major per XCD
+    const int64_t ng = (prm.n + BWD_GROUP - 1) / BWD_GROUP;
+    const int64_t grid = 8 * ng * ((ng + 7) / 8);
+    if (grid > 0x7fffffff)
+      fail(SBAMD_ERR_INVALID, "backward: too many PCFs for one launch");
+    const size_t lds = bwd_lds_bytes(prm.w_f64 ? 8 : 4);
+    const dim3 g(static_cast<unsigned>(grid));
+    const dim3 blk(64 * BWD_WAVES);
+    if (prm.w_f64)
+    {
+      if (prm.op == OPK_DOT)
+        hipLaunchKernelGGL((bwd_stream_kernel<double, OPK_DOT>), g, blk, lds, stream, prm);
       else
-        hipLaunchKernelGGL((bwd_tile_kernel<double, OPK_L2>), dim3(grid), block, lds, stream, prm);
+        hipLaunchKernelGGL((bwd_stream_kernel<double, OPK_L2>), g, blk, lds, stream, prm);
     }
     else
     {
-      if (op == OPK_DOT)
-        hipLaunchKernelGGL((bwd_tile_kernel<float, OPK_DOT>), dim3(grid), block, lds, stream, prm);
+      if (prm.op == OPK_DOT)
+        hipLaunchKernelGGL((bwd_stream_kernel<float, OPK_DOT>), g, blk, lds, stream, prm);
       else
-        hipLaunchKernelGGL((bwd_tile_kernel<float, OPK_L2>), dim3(grid), block, lds, stream, prm);
+        hipLaunchKernelGGL((bwd_stream_kernel<float, OPK_L2>), g, blk, lds, stream, prm);
     }
     SBAMD_HIP(hipGetLastError());
-  }
-
-  void launch_finalize_grad(const FinalizeParams& prm, hipStream_t stream)
-  {
-    if (prm.records <= 0)
-      return;
-    const unsigned grid = static_cast<unsigned>((prm.records + 255) / 256);
-    hipLaunchKernelGGL(finalize_grad_kernel, dim3(grid), dim3(256), 0, stream, prm);
+    hipLaunchKernelGGL(finalize_grad_kernel, dim3(rgrid), dim3(256), 0, stream, prm);
     SBAMD_HIP(hipGetLastError());
   }
 }

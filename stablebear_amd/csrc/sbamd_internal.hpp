@@ -116,34 +116,56 @@ namespace sbamd
     int32_t pad;
   };
 
+  // One event of a group's merged breakpoint list (a group is 64 consecutive
+  // PCFs): the record at position pos starts at time t, dt after the group's
+  // previous event, and the PCF's value jumps there by d. col is the PCF's
+  // index within the group, with BWD_FIRST set on a PCF's first record.
+  struct BwdEvent
+  {
+    double t;
+    double dt;
+    double d;
+    int32_t col;
+    int32_t pos;
+  };
+  constexpr int32_t BWD_GROUP = 64;
+  constexpr int32_t BWD_FIRST = 0x40000000;
+  static_assert(sizeof(BwdEvent) == 32, "BwdEvent must stay 32 bytes");
+
+  // Parameters of the backward kernels (kernels_bwd.hip). Arrays marked
+  // "position" are in the original PCF order: record k of PCF i at ooff[i] + k.
   struct BwdParams
   {
-    FwdParams f;          // forward geometry (out unused)
-    const void* grad_out; // upstream gradient per output element
-    const void* fwd_out;  // forward output (distances) for p = 2
-    double* q;            // per record: sum over pairs of w * integral of the partner over the record
-    double* a;            // per record: sum over pairs of w * partner value at the record's end
-    double* wsum;         // per PCF (sorted position): sum over its pairs of w
-    int32_t in_f64;       // grad_out / fwd_out are FP64
-    int32_t val_f64;      // record values are FP64 (else FP32); record times are FP64
-  };
-
-  struct FinalizeParams
-  {
-    const int64_t* roff; // sorted record offsets (n + 1)
-    const void* rec;     // records {t_end (FP64), value (FP64 or FP32)}, 16 bytes each
-    const int64_t* src;  // record -> original point
-    const double* q;
-    const double* a;
-    const double* wsum;
+    int64_t n;       // PCFs
+    int64_t records; // records of the set (= positions)
+    int32_t ch;      // PCFs per group (BWD_GROUP)
+    int32_t op;      // OPK_L2 (pdist) or OPK_DOT (kernel with diagonal)
+    int32_t in_f64;  // grad_out / fwd_out are FP64
+    int32_t w_f64;   // pair weights kept in FP64 (else FP32) in LDS
+    int32_t val_f64; // record values are FP64 (else FP32); record times are FP64
+    int32_t grad_f64;
+    // the sorted device set
+    const int64_t* roff;
+    const void* rec;
+    const int64_t* perm;
+    const int64_t* src;
+    // position layout
+    const int64_t* ooff; // n + 1
+    double* key0;        // record start time
+    double* val0;        // record value
+    int32_t* col0;       // owning PCF
+    int64_t* src0;       // source point (-1: zero record of an empty PCF)
+    double* keys[2];     // merge sort ping-pong
+    int32_t* ids[2];
+    BwdEvent* events;
+    // loss and accumulators
+    const void* grad_out;
+    const void* fwd_out;
+    double* q;    // per position: integral of H over the record
+    double* a;    // per position: H where the record ends
+    double* wsum; // per PCF: sum of its pair weights
     void* grad_v;
     void* grad_t;
-    int64_t n;
-    int64_t records;
-    int32_t val_f64;
-    int32_t grad_f64;
-    int32_t op;
-    int32_t pad;
   };
 
   // ---------------------------------------------------------------------
@@ -230,9 +252,9 @@ namespace sbamd
     // persistent launch: resident workgroups pull tiles from this counter
     bool persistent = false;
     DeviceBuffer counter;
-    // backward accumulators (per record q, a; per PCF wsum), allocated on first use
-    DeviceBuffer bwd_q, bwd_a, bwd_w;
-    int32_t bwd_cap = 32; // records staged per slot and phase in the backward
+    // backward working arrays (kernels_bwd.hip), allocated on first use
+    DeviceBuffer bwd_ooff, bwd_key0, bwd_val0, bwd_col0, bwd_src0, bwd_keys[2], bwd_ids[2],
+        bwd_events, bwd_q, bwd_a, bwd_w;
     // long-pair kernel: mode 0 off, 1 on, 2 auto (few long pairs, FP32C);
     // use_long is the decision for this plan
     int32_t long_mode = 2;
@@ -248,9 +270,7 @@ namespace sbamd
   void launch_long(const LongParams& prm, int tt_f64, int tv_f64, int op, int kahan, int cu_count,
                    hipStream_t stream);
   void launch_norm(const NormParams& prm, int tt_f64, int tv_f64, hipStream_t stream);
-  void launch_backward(const BwdParams& prm, int op, hipStream_t stream);
-  void launch_finalize_grad(const FinalizeParams& prm, hipStream_t stream);
-  size_t bwd_lds_bytes(int tv_f64, int R, int C, int cap);
+  void launch_backward(const BwdParams& prm, hipStream_t stream);
   void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream);
   void launch_rectangles(const double* ft, const double* fv, int64_t mf, const double* gt,
                          const double* gv, int64_t mg, double a, double b, double* out,

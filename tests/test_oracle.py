@@ -359,3 +359,24 @@ def test_gradient_closed_forms_constant_and_self_pair():
     # gives dS/dT = (a - b)^2 = 9 and dD/dT = 9 / (2 D) = 0.5; the two one-sided
     # halves of the tie convention add up to it.
     assert gt[1] + gt[3] == pytest.approx(0.5)
+
+
+def test_gradient_tie_convention_is_the_mean_and_order_free():
+    """At a shared breakpoint each side sees the mean of the other side's two
+    limits: f = 4 -> 0 and g = 1 -> 0 at T = 9 give dS/dT_f = (4 - s)^2 - (0 - s)^2
+    with s = 0.5, and the result does not depend on which PCF comes first."""
+    A, Bp = [[0.0, 4.0], [9.0, 0.0]], [[0.0, 1.0], [9.0, 0.0]]
+    off, t, v = _csr([A, Bp])
+    _, gt = oracle.grad(oracle.OP_LP, oracle.KIND_PDIST, off, t, v, np.array([1.0]))
+    D = 3.0 * 3.0
+    assert gt[1] == pytest.approx(((4 - 0.5) ** 2 - 0.5 ** 2) / (2 * D))
+    assert gt[3] == pytest.approx(((1 - 2.0) ** 2 - 2.0 ** 2) / (2 * D))
+    off2, t2, v2 = _csr([Bp, A])
+    _, gt2 = oracle.grad(oracle.OP_LP, oracle.KIND_PDIST, off2, t2, v2, np.array([1.0]))
+    assert gt2[3] == pytest.approx(gt[1]) and gt2[1] == pytest.approx(gt[3])
+    # a partner jumping at the same time in the inner product: (a - b) * mean
+    off3, t3, v3 = _csr([[[0.0, 2.0], [1.0, 5.0], [3.0, 0.0]], [[0.0, -1.0], [1.0, 3.0], [4.0, 0.0]]])
+    G3 = np.array([0.0, 1.0, 0.0])  # only K_01
+    _, gt3 = oracle.grad(oracle.OP_DOT, oracle.KIND_KERNEL, off3, t3, v3, G3)
+    assert gt3[1] == pytest.approx((2.0 - 5.0) * 0.5 * (-1.0 + 3.0))
+    assert gt3[4] == pytest.approx((-1.0 - 3.0) * 0.5 * (2.0 + 5.0))
