@@ -1005,6 +1005,8 @@ extern "C"
       B.w_f64 = plan->tv_f64; // FP64 sets keep FP64 pair weights
       B.val_f64 = plan->tv_f64;
       B.grad_f64 = grad_dtype == SBAMD_F64 ? 1 : 0;
+      // chunks of 64 partners: 16 KB (FP32) or 32 KB (FP64) of weights in
+      // LDS, so that 16-wave workgroups still fill a CU
       B.ch = BWD_GROUP;
       B.roff = s->roff.as<int64_t>();
       B.rec = s->rec.ptr;
@@ -1021,6 +1023,7 @@ extern "C"
         B.ids[b] = plan->bwd_ids[b].as<int32_t>();
       }
       B.events = plan->bwd_events.as<BwdEvent>();
+      B.events2 = B.events; // chunks are the same groups of 64
       B.grad_out = grad_out_device;
       B.fwd_out = fwd_out_device;
       B.q = plan->bwd_q.as<double>();

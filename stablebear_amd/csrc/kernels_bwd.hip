@@ -124,20 +124,20 @@ namespace sbamd
     }
   }
 
-  constexpr int BWD_WAVES = 4; // wavefronts per backward workgroup (time slices of one task)
+  constexpr int BWD_WAVES = 16; // wavefronts per backward workgroup (time slices of one task)
   constexpr int BWD_BLK = 32;  // events per staged LDS block of one list
 
-  // LDS bytes of the pair weights and of the whole workgroup: weights, and
-  // per wave the partners' start values (64 FP64), two staged blocks and 64
-  // pending emissions (Q, A, record).
-  __host__ __device__ constexpr size_t bwd_lds_weights(size_t wbytes)
+  // LDS bytes of the pair weights (ch partners x 64 rows) and of the whole
+  // workgroup: weights, and per wave the partners' start values (ch FP64),
+  // two staged blocks and 64 pending emissions (Q, A, record).
+  __host__ __device__ constexpr size_t bwd_lds_weights(size_t wbytes, int ch)
   {
-    return static_cast<size_t>(BWD_GROUP) * 64 * wbytes;
+    return static_cast<size_t>(ch) * 64 * wbytes;
   }
-  __host__ __device__ constexpr size_t bwd_lds_bytes(size_t wbytes)
+  __host__ __device__ constexpr size_t bwd_lds_bytes(size_t wbytes, int ch)
   {
-    return bwd_lds_weights(wbytes) + BWD_WAVES * (64 * sizeof(double) + 2 * BWD_BLK * sizeof(BwdEvent) +
-                                                  2 * 64 * sizeof(double) + 64 * sizeof(int32_t));
+    return bwd_lds_weights(wbytes, ch) + BWD_WAVES * (ch * sizeof(double) + 2 * BWD_BLK * sizeof(BwdEvent) +
+                                                      2 * 64 * sizeof(double) + 64 * sizeof(int32_t));
   }
 
   // Thread per sorted record: copies start time, value, owner and source
@@ -174,7 +174,7 @@ namespace sbamd
   // One level of the per-chunk merge sort: runs of `width` consecutive PCFs
   // (each PCF's records are already in time order) merge pairwise; equal
   // times keep the left run first, so the order is (time, PCF, record).
-  __global__ void bwd_merge_kernel(BwdParams B, int32_t width, int32_t from)
+  __global__ void bwd_merge_kernel(BwdParams B, int32_t width, int32_t from, int32_t group)
   {
     const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= B.records)
@@ -182,8 +182,8 @@ namespace sbamd
     const double* kin = B.keys[from];
     const int32_t* iin = B.ids[from];
     const int64_t i = B.col0[e];
-    const int64_t cb = (i / B.ch) * B.ch;
-    const int64_t ce = min(cb + B.ch, B.n);
+    const int64_t cb = (i / group) * group;
+    const int64_t ce = min(cb + group, B.n);
     const int64_t run = (i - cb) / width;
     const int64_t sib = run ^ 1;
     const int64_t own0 = cb + run * width;
@@ -201,12 +201,12 @@ namespace sbamd
   }
 
   // Thread per position of the sorted chunk lists: the event record.
-  __global__ void bwd_events_kernel(BwdParams B, int32_t from)
+  __global__ void bwd_events_kernel(BwdParams B, int32_t from, int32_t group, BwdEvent* out)
   {
     const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= B.records)
       return;
-    const int64_t cb = (static_cast<int64_t>(B.col0[p]) / B.ch) * B.ch;
+    const int64_t cb = (static_cast<int64_t>(B.col0[p]) / group) * group;
     const double t = B.keys[from][p];
     const double tp = p > B.ooff[cb] ? B.keys[from][p - 1] : 0.0;
     const int64_t id = B.ids[from][p];
@@ -218,25 +218,28 @@ namespace sbamd
     ev.d = B.val0[id] - (first ? 0.0 : B.val0[id - 1]);
     ev.col = static_cast<int32_t>(c - cb) | (first ? BWD_FIRST : 0);
     ev.pos = static_cast<int32_t>(id);
-    B.events[p] = ev;
+    out[p] = ev;
   }
 
-  // One workgroup per (band of 64 rows, chunk of 64 partners), four
+  // One workgroup per (band of 64 rows, chunk of CH partners), four
   // wavefronts splitting the task's time axis: wave k takes the events in
   // [T_k, T_{k+1}), with T_k the time of the chunk's event at k/4 of its
   // list. A wave starts from H_i just before T_k (from the partners' values
   // there) and from the row record open there; the part of that record's
   // integral before T_k was added by wave k - 1. Each wave stages both lists
   // in LDS blocks of BWD_BLK events and reads them at wave-uniform
-  // addresses. Workgroups are dealt to the 8 XCDs round-robin; all bands of a
-  // chunk run on one XCD so the chunk's list stays in that XCD's L2.
-  template <typename WT, int OP>
+  // addresses. A breakpoint event costs one lane's work and a partner event
+  // all lanes', so wider chunks (CH = 128 with FP32 weights) spend a smaller
+  // share of the events on breakpoints. Workgroups are dealt to the 8 XCDs
+  // round-robin; all bands of a chunk run on one XCD so the chunk's list
+  // stays in that XCD's L2.
+  template <typename WT, int OP, int CH>
   __global__ __launch_bounds__(64 * BWD_WAVES) void bwd_stream_kernel(BwdParams B)
   {
     extern __shared__ __align__(16) unsigned char smem[];
     WT* const s_w = reinterpret_cast<WT*>(smem); // s_w[jj * 64 + ii] = w(row ii, partner jj)
-    double* const s_g = reinterpret_cast<double*>(smem + bwd_lds_weights(sizeof(WT)));
-    BwdEvent* const s_ev = reinterpret_cast<BwdEvent*>(s_g + BWD_WAVES * 64);
+    double* const s_g = reinterpret_cast<double*>(smem + bwd_lds_weights(sizeof(WT), CH));
+    BwdEvent* const s_ev = reinterpret_cast<BwdEvent*>(s_g + BWD_WAVES * CH);
     double* const s_em = reinterpret_cast<double*>(s_ev + BWD_WAVES * 2 * BWD_BLK);
     int32_t* const s_ip = reinterpret_cast<int32_t*>(s_em + BWD_WAVES * 2 * 64);
     const int tid = threadIdx.x;
@@ -244,22 +247,23 @@ namespace sbamd
     // the wave index, as a wave-uniform value, so that everything derived
     // from it (slice bounds, list indices) stays in scalar registers
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int64_t ng = (B.n + BWD_GROUP - 1) / BWD_GROUP;
+    const int64_t nbands = (B.n + BWD_GROUP - 1) / BWD_GROUP;
+    const int64_t nchunks = (B.n + CH - 1) / CH;
     const int64_t x = blockIdx.x;
     const int64_t slot = x >> 3;
-    const int64_t c = (x & 7) + 8 * (slot / ng);
-    if (c >= ng)
+    const int64_t c = (x & 7) + 8 * (slot / nbands);
+    if (c >= nchunks)
       return;
-    const int64_t band = slot % ng;
+    const int64_t band = slot % nbands;
     const int64_t i = band * BWD_GROUP + lane;
     const bool act = i < B.n;
-    const int64_t j0 = c * BWD_GROUP;
-    const int ncol = static_cast<int>(min(static_cast<int64_t>(BWD_GROUP), B.n - j0));
+    const int64_t j0 = c * CH;
+    const int ncol = static_cast<int>(min(static_cast<int64_t>(CH), B.n - j0));
     const int nrow = static_cast<int>(min(static_cast<int64_t>(BWD_GROUP), B.n - band * BWD_GROUP));
 
     // pair weights, partner-major: the lanes read one row of 64 consecutive
     // words per event
-    for (int k = tid; k < BWD_GROUP * 64; k += 64 * BWD_WAVES)
+    for (int k = tid; k < CH * 64; k += 64 * BWD_WAVES)
     {
       const int jj = k >> 6, ii = k & 63;
       const double w = jj < ncol && ii < nrow ? pair_weight<OP>(B, band * BWD_GROUP + ii, j0 + jj) : 0.0;
@@ -281,7 +285,7 @@ namespace sbamd
     const int64_t c0 = uniform_load(B.ooff, j0);
     const int32_t ncv = static_cast<int32_t>(uniform_load(B.ooff, j0 + ncol) - c0);
     const BwdEvent* const rl = B.events + r0;
-    const BwdEvent* const cl = B.events + c0;
+    const BwdEvent* const cl = B.events2 + c0;
     const bool last = wv == BWD_WAVES - 1;
     double tlo = 0.0, thi = __builtin_inf();
     int32_t cb = 0, ce = ncv, rb = 0, re = nr;
@@ -302,17 +306,21 @@ namespace sbamd
     double H = 0.0;
     if (wv > 0)
     {
-      double gv = 0.0;
-      if (lane < ncol)
+      for (int jb = 0; jb < CH; jb += 64)
       {
-        const int64_t a = B.ooff[j0 + lane];
-        const int64_t k = starts_below(B.key0, a, B.ooff[j0 + lane + 1], tlo);
-        if (k > 0)
-          gv = B.val0[a + k - 1];
+        const int jj = jb + lane;
+        double gv = 0.0;
+        if (jj < ncol)
+        {
+          const int64_t a = B.ooff[j0 + jj];
+          const int64_t k = starts_below(B.key0, a, B.ooff[j0 + jj + 1], tlo);
+          if (k > 0)
+            gv = B.val0[a + k - 1];
+        }
+        s_g[wv * CH + jj] = gv;
       }
-      s_g[wv * 64 + lane] = gv;
       for (int jj = 0; jj < ncol; ++jj)
-        H = fma(static_cast<double>(s_w[jj * 64 + lane]), s_g[wv * 64 + jj], H);
+        H = fma(static_cast<double>(s_w[jj * 64 + lane]), s_g[wv * CH + jj], H);
     }
     // the row record open just before tlo
     int64_t open = 0;
@@ -376,7 +384,7 @@ namespace sbamd
       // equal times, so H is the right limit at a breakpoint)
       while (hc && C.t <= Rt)
       {
-        const double w = static_cast<double>(s_w[(C.col & (BWD_GROUP - 1)) * 64 + lane]);
+        const double w = static_cast<double>(s_w[(C.col & (CH - 1)) * 64 + lane]);
         acc = fma(H, C.t - tprev, acc);
         H = fma(w, C.d, H);
         tprev = C.t;
@@ -409,7 +417,7 @@ namespace sbamd
             const BwdEvent T = e >= cs ? bufC[e - cs] : uniform_event(cl, e);
             if (__double_as_longlong(T.t) != __double_as_longlong(tprev))
               break;
-            J = fma(static_cast<double>(s_w[(T.col & (BWD_GROUP - 1)) * 64 + lane]), T.d, J);
+            J = fma(static_cast<double>(s_w[(T.col & (CH - 1)) * 64 + lane]), T.d, J);
           }
         }
         if (lane == R.col)
@@ -497,43 +505,61 @@ namespace sbamd
   {
     if (prm.records <= 0 || prm.n <= 0)
       return;
-    if (prm.ch != BWD_GROUP)
-      fail(SBAMD_ERR_RUNTIME, "backward: group size must be 64");
+    if (prm.ch != BWD_GROUP && !(prm.ch == 2 * BWD_GROUP && !prm.w_f64))
+      fail(SBAMD_ERR_RUNTIME, "backward: chunks of 64 partners, or 128 with FP32 weights");
     const unsigned rgrid = static_cast<unsigned>((prm.records + 255) / 256);
     hipLaunchKernelGGL(bwd_layout_kernel, dim3(rgrid), dim3(256), 0, stream, prm);
     SBAMD_HIP(hipGetLastError());
+    // merge levels up to groups of 64 (the bands' lists), then one more
+    // level for chunks of 128
     int from = 0;
     for (int32_t width = 1; width < prm.ch; width *= 2)
     {
-      hipLaunchKernelGGL(bwd_merge_kernel, dim3(rgrid), dim3(256), 0, stream, prm, width, from);
+      const int32_t group = width < BWD_GROUP ? BWD_GROUP : prm.ch;
+      if (width == BWD_GROUP)
+      {
+        hipLaunchKernelGGL(bwd_events_kernel, dim3(rgrid), dim3(256), 0, stream, prm, from, BWD_GROUP,
+                           prm.events);
+        SBAMD_HIP(hipGetLastError());
+      }
+      hipLaunchKernelGGL(bwd_merge_kernel, dim3(rgrid), dim3(256), 0, stream, prm, width, from, group);
       SBAMD_HIP(hipGetLastError());
       from ^= 1;
     }
-    hipLaunchKernelGGL(bwd_events_kernel, dim3(rgrid), dim3(256), 0, stream, prm, from);
+    hipLaunchKernelGGL(bwd_events_kernel, dim3(rgrid), dim3(256), 0, stream, prm, from, prm.ch,
+                       prm.events2);
     SBAMD_HIP(hipGetLastError());
 
     // (band, chunk) workgroups: every band of 64 rows against every chunk of
-    // 64 partners, chunk-major per XCD
-    const int64_t ng = (prm.n + BWD_GROUP - 1) / BWD_GROUP;
-    const int64_t grid = 8 * ng * ((ng + 7) / 8);
+    // ch partners, chunk-major per XCD
+    const int64_t nbands = (prm.n + BWD_GROUP - 1) / BWD_GROUP;
+    const int64_t nchunks = (prm.n + prm.ch - 1) / prm.ch;
+    const int64_t grid = 8 * nbands * ((nchunks + 7) / 8);
     if (grid > 0x7fffffff)
       fail(SBAMD_ERR_INVALID, "backward: too many PCFs for one launch");
-    const size_t lds = bwd_lds_bytes(prm.w_f64 ? 8 : 4);
+    const size_t lds = bwd_lds_bytes(prm.w_f64 ? 8 : 4, prm.ch);
     const dim3 g(static_cast<unsigned>(grid));
     const dim3 blk(64 * BWD_WAVES);
     if (prm.w_f64)
     {
       if (prm.op == OPK_DOT)
-        hipLaunchKernelGGL((bwd_stream_kernel<double, OPK_DOT>), g, blk, lds, stream, prm);
+        hipLaunchKernelGGL((bwd_stream_kernel<double, OPK_DOT, BWD_GROUP>), g, blk, lds, stream, prm);
       else
-        hipLaunchKernelGGL((bwd_stream_kernel<double, OPK_L2>), g, blk, lds, stream, prm);
+        hipLaunchKernelGGL((bwd_stream_kernel<double, OPK_L2, BWD_GROUP>), g, blk, lds, stream, prm);
+    }
+    else if (prm.ch == 2 * BWD_GROUP)
+    {
+      if (prm.op == OPK_DOT)
+        hipLaunchKernelGGL((bwd_stream_kernel<float, OPK_DOT, 2 * BWD_GROUP>), g, blk, lds, stream, prm);
+      else
+        hipLaunchKernelGGL((bwd_stream_kernel<float, OPK_L2, 2 * BWD_GROUP>), g, blk, lds, stream, prm);
     }
     else
     {
       if (prm.op == OPK_DOT)
-        hipLaunchKernelGGL((bwd_stream_kernel<float, OPK_DOT>), g, blk, lds, stream, prm);
+        hipLaunchKernelGGL((bwd_stream_kernel<float, OPK_DOT, BWD_GROUP>), g, blk, lds, stream, prm);
       else
-        hipLaunchKernelGGL((bwd_stream_kernel<float, OPK_L2>), g, blk, lds, stream, prm);
+        hipLaunchKernelGGL((bwd_stream_kernel<float, OPK_L2, BWD_GROUP>), g, blk, lds, stream, prm);
     }
     SBAMD_HIP(hipGetLastError());
     hipLaunchKernelGGL(finalize_grad_kernel, dim3(rgrid), dim3(256), 0, stream, prm);

@@ -138,7 +138,7 @@ namespace sbamd
   {
     int64_t n;       // PCFs
     int64_t records; // records of the set (= positions)
-    int32_t ch;      // PCFs per group (BWD_GROUP)
+    int32_t ch;      // partners per chunk (BWD_GROUP or 2 * BWD_GROUP)
     int32_t op;      // OPK_L2 (pdist) or OPK_DOT (kernel with diagonal)
     int32_t in_f64;  // grad_out / fwd_out are FP64
     int32_t w_f64;   // pair weights kept in FP64 (else FP32) in LDS
@@ -157,7 +157,8 @@ namespace sbamd
     int64_t* src0;       // source point (-1: zero record of an empty PCF)
     double* keys[2];     // merge sort ping-pong
     int32_t* ids[2];
-    BwdEvent* events;
+    BwdEvent* events;  // event lists of the groups of BWD_GROUP PCFs (the bands)
+    BwdEvent* events2; // event lists of the chunks of ch PCFs (== events when ch == BWD_GROUP)
     // loss and accumulators
     const void* grad_out;
     const void* fwd_out;
