@@ -15,13 +15,14 @@
 // breakpoint, by w_ij times that partner's jump. PCFs are taken in groups of
 // 64 consecutive ones; each group's record starts, merged into one
 // time-ordered event list, serve twice. A wavefront handles a band of 64
-// rows (one lane per row) against a chunk of 64 partners by merging two
-// wave-uniform event lists staged in LDS: the chunk's list, where every
-// event moves all lanes' H_i (w_ij from LDS), and the band's own list, where
-// an event is one lane's breakpoint: that lane emits the Q and A of the
-// record that ends there. Control flow stays uniform, nothing is shared
-// between lanes, and the cost is the two lists' length; the chunks of a band
-// add their Q and A with global atomics.
+// rows (one lane per row) against a chunk of 64 partners: it walks the
+// chunk's list, where every event moves all lanes' H_i (w_ij from LDS), and
+// before each event every lane emits the Q and A of its own records that
+// end before that event's time, all lanes at once (the band's list, sorted
+// into per-lane queues in LDS, supplies each lane's breakpoints). The cost
+// is the chunk list's length plus one emission round per distinct gap
+// holding breakpoints; the chunks of a band add their Q and A with global
+// atomics, 64 records at a time.
 //
 // All arrays here are in the original PCF order ("position" p of record k of
 // PCF i is ooff[i] + k); a layout kernel copies them out of the sorted
@@ -125,19 +126,23 @@ namespace sbamd
   }
 
   constexpr int BWD_WAVES = 16; // wavefronts per backward workgroup (time slices of one task)
-  constexpr int BWD_BLK = 32;  // events per staged LDS block of one list
+
+  // Per wave LDS: four arrays of 64 8-byte words (row queue times, pending
+  // Q, pending A, row masks) and three of 64 ints (row queue positions,
+  // pending records, queue offsets). The partners' start values (ch FP64,
+  // needed only before the main loop) share the first two 8-byte arrays.
+  constexpr size_t BWD_WAVE_LDS = 4 * 64 * sizeof(double) + 3 * 64 * sizeof(int32_t);
+  static_assert(BWD_WAVE_LDS >= 128 * sizeof(double), "start values of 128 partners must fit");
 
   // LDS bytes of the pair weights (ch partners x 64 rows) and of the whole
-  // workgroup: weights, and per wave the partners' start values (ch FP64),
-  // two staged blocks and 64 pending emissions (Q, A, record).
+  // workgroup.
   __host__ __device__ constexpr size_t bwd_lds_weights(size_t wbytes, int ch)
   {
     return static_cast<size_t>(ch) * 64 * wbytes;
   }
   __host__ __device__ constexpr size_t bwd_lds_bytes(size_t wbytes, int ch)
   {
-    return bwd_lds_weights(wbytes, ch) + BWD_WAVES * (ch * sizeof(double) + 2 * BWD_BLK * sizeof(BwdEvent) +
-                                                      2 * 64 * sizeof(double) + 64 * sizeof(int32_t));
+    return bwd_lds_weights(wbytes, ch) + BWD_WAVES * BWD_WAVE_LDS;
   }
 
   // Thread per sorted record: copies start time, value, owner and source
@@ -221,40 +226,50 @@ namespace sbamd
     out[p] = ev;
   }
 
-  // One workgroup per (band of 64 rows, chunk of CH partners), four
+  // One workgroup per (band of 64 rows, chunk of CH partners), BWD_WAVES
   // wavefronts splitting the task's time axis: wave k takes the events in
-  // [T_k, T_{k+1}), with T_k the time of the chunk's event at k/4 of its
-  // list. A wave starts from H_i just before T_k (from the partners' values
-  // there) and from the row record open there; the part of that record's
-  // integral before T_k was added by wave k - 1. Each wave stages both lists
-  // in LDS blocks of BWD_BLK events and reads them at wave-uniform
-  // addresses. A breakpoint event costs one lane's work and a partner event
+  // [T_k, T_{k+1}), with T_k the time of the chunk's event at k/BWD_WAVES of
+  // its list. A wave starts from H_i just before T_k (from the partners'
+  // values there) and from the row record open there; the part of that
+  // record's integral before T_k was added by wave k - 1. Each wave holds
+  // blocks of 64 events of both lists in vector registers (lane l holds
+  // event l) and reads them with readlane. A breakpoint event costs one lane's work and a partner event
   // all lanes', so wider chunks (CH = 128 with FP32 weights) spend a smaller
   // share of the events on breakpoints. Workgroups are dealt to the 8 XCDs
   // round-robin; all bands of a chunk run on one XCD so the chunk's list
-  // stays in that XCD's L2.
+  // stays in that XCD's L2. Chunks go to XCDs in snake order (0..7, 7..0,
+  // ...) so that every XCD gets a like share of light and heavy chunks.
   template <typename WT, int OP, int CH>
   __global__ __launch_bounds__(64 * BWD_WAVES) void bwd_stream_kernel(BwdParams B)
   {
     extern __shared__ __align__(16) unsigned char smem[];
     WT* const s_w = reinterpret_cast<WT*>(smem); // s_w[jj * 64 + ii] = w(row ii, partner jj)
-    double* const s_g = reinterpret_cast<double*>(smem + bwd_lds_weights(sizeof(WT), CH));
-    BwdEvent* const s_ev = reinterpret_cast<BwdEvent*>(s_g + BWD_WAVES * CH);
-    double* const s_em = reinterpret_cast<double*>(s_ev + BWD_WAVES * 2 * BWD_BLK);
-    int32_t* const s_ip = reinterpret_cast<int32_t*>(s_em + BWD_WAVES * 2 * 64);
+    unsigned char* const s_wave = smem + bwd_lds_weights(sizeof(WT), CH);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     // the wave index, as a wave-uniform value, so that everything derived
     // from it (slice bounds, list indices) stays in scalar registers
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    double* const s_qt = reinterpret_cast<double*>(s_wave + wv * BWD_WAVE_LDS); // row queue: times
+    double* const s_eq = s_qt + 64;                                              // pending Q
+    double* const s_ea = s_eq + 64;                                              // pending A
+    unsigned long long* const s_m = reinterpret_cast<unsigned long long*>(s_ea + 64); // row masks
+    int32_t* const s_qp = reinterpret_cast<int32_t*>(s_m + 64);                  // row queue: positions
+    int32_t* const s_ep = s_qp + 64;                                             // pending records
+    int32_t* const s_off = s_ep + 64;                                            // queue offsets
+    double* const s_g = s_qt; // partners' values at tlo (before the main loop only)
     const int64_t nbands = (B.n + BWD_GROUP - 1) / BWD_GROUP;
     const int64_t nchunks = (B.n + CH - 1) / CH;
     const int64_t x = blockIdx.x;
     const int64_t slot = x >> 3;
-    const int64_t c = (x & 7) + 8 * (slot / nbands);
-    if (c >= nchunks)
+    const int64_t g = slot / nbands;
+    const int64_t cr = ((g & 1) ? 7 - (x & 7) : (x & 7)) + 8 * g;
+    if (cr >= nchunks)
       return;
-    const int64_t band = slot % nbands;
+    // PCFs are sorted by record count, so the last chunks and bands are the
+    // heaviest tasks: they go first
+    const int64_t c = nchunks - 1 - cr;
+    const int64_t band = nbands - 1 - slot % nbands;
     const int64_t i = band * BWD_GROUP + lane;
     const bool act = i < B.n;
     const int64_t j0 = c * CH;
@@ -317,10 +332,10 @@ namespace sbamd
           if (k > 0)
             gv = B.val0[a + k - 1];
         }
-        s_g[wv * CH + jj] = gv;
+        s_g[jj] = gv;
       }
       for (int jj = 0; jj < ncol; ++jj)
-        H = fma(static_cast<double>(s_w[jj * 64 + lane]), s_g[wv * CH + jj], H);
+        H = fma(static_cast<double>(s_w[jj * 64 + lane]), s_g[jj], H);
     }
     // the row record open just before tlo
     int64_t open = 0;
@@ -338,111 +353,190 @@ namespace sbamd
         Hend = fma(static_cast<double>(s_w[jj * 64 + lane]),
                    B.val0[uniform_load(B.ooff, j0 + jj + 1) - 1], Hend);
 
-    // Pending emissions (record, Q, A) go out 64 at a time, one lane per
-    // record. A flush follows a staged block's store, so no load of the
-    // event lists ever waits for these atomics (vector memory operations
-    // complete in order).
-    double* const em_q = s_em + wv * 2 * 64;
-    double* const em_a = em_q + 64;
-    int32_t* const em_p = s_ip + wv * 64;
+    // A block of 64 events of a list in vector registers: lane l holds
+    // event s + l (index clamped into the list, so that every register is
+    // defined). Only the fields used are loaded: a load whose destination is
+    // never read leaves a register that the compiler reuses, and that reuse
+    // waits for the load.
+    struct PartnerBlk
+    {
+      double t, d;
+      int32_t col;
+    };
+    auto load_partners = [&](int32_t s0) -> PartnerBlk {
+      const BwdEvent* const ev = cl + max(min(s0 + lane, ncv - 1), 0);
+      PartnerBlk b;
+      b.t = ev->t;
+      b.d = ev->d;
+      b.col = ev->col;
+      return b;
+    };
+    auto lane_d = [](double v, int l) -> double {
+      return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                              __builtin_amdgcn_readlane(__double2loint(v), l));
+    };
+
+    // Row breakpoints are taken by their own lanes, all lanes at once. The
+    // wave holds a block of up to 64 events of the band's list, sorted by
+    // lane into per-lane queues in LDS (s_qt / s_qp, lane L's events at
+    // [qptr, qend) in time order): a 64-bit mask per lane (bit e set when
+    // event e of the block ends one of its records, built with LDS atomic
+    // ORs) gives each event its rank in its lane's queue. nb is the lane's
+    // next breakpoint and npos the position of the record that starts
+    // there; a lane with no events left in the block has nb = the block's
+    // last time, so that a partner event past that time brings in the next
+    // block. The row block is loaded when it is needed, not ahead: a block
+    // held in registers across the partner loop costs register copies there
+    // that wait on every outstanding memory operation.
+    int32_t rs = rb;
+    int32_t qptr = 0, qend = 0;
+    double nb = 0.0, blast = 0.0;
+    int32_t npos = 0;
+    auto next_own = [&]() {
+      const int32_t k = min(qptr, 63);
+      const double t = s_qt[k];
+      npos = s_qp[k];
+      nb = qptr < qend ? t : blast;
+    };
+    auto open_block = [&]() {
+      const BwdEvent* const ev = rl + max(min(rs + lane, nr - 1), 0);
+      const double t = ev->t;
+      const int32_t col = ev->col & 63, pos = ev->pos;
+      const bool valid = rs + lane < re && !(ev->col & BWD_FIRST);
+      s_m[lane] = 0ull;
+      __builtin_amdgcn_wave_barrier();
+      if (valid)
+        atomicOr(&s_m[col], 1ull << lane);
+      __builtin_amdgcn_wave_barrier();
+      // queue offsets: exclusive prefix sum of the lanes' event counts
+      const int32_t cnt = __popcll(s_m[lane]);
+      int32_t incl = cnt;
+      for (int o = 1; o < 64; o <<= 1)
+      {
+        const int32_t v = __shfl_up(incl, o);
+        if (lane >= o)
+          incl += v;
+      }
+      qptr = incl - cnt;
+      qend = incl;
+      s_off[lane] = qptr;
+      __builtin_amdgcn_wave_barrier();
+      if (valid)
+      {
+        const int32_t slot = s_off[col] + static_cast<int32_t>(__popcll(s_m[col] & ((1ull << lane) - 1ull)));
+        s_qt[slot] = t;
+        s_qp[slot] = pos;
+      }
+      __builtin_amdgcn_wave_barrier();
+      blast = rs + 64 < re ? lane_d(t, 63) : __builtin_inf();
+      next_own();
+    };
+    open_block();
+
+    // Pending emissions (record, Q, A), added to q and a 64 at a time.
     int32_t nem = 0;
     auto flush = [&]() {
+      __builtin_amdgcn_wave_barrier();
       if (lane < nem)
       {
-        atomicAdd(&B.q[em_p[lane]], em_q[lane]);
-        atomicAdd(&B.a[em_p[lane]], em_a[lane]);
+        atomicAdd(&B.q[s_ep[lane]], s_eq[lane]);
+        atomicAdd(&B.a[s_ep[lane]], s_ea[lane]);
       }
       nem = 0;
     };
-    BwdEvent* const bufC = s_ev + wv * 2 * BWD_BLK;
-    BwdEvent* const bufR = bufC + BWD_BLK;
-    // Every lane loads and stores (index clamped; lanes 32..63 repeat lanes
-    // 0..31, slots past the list end are never read), so no loaded register
-    // stays pending past the store.
-    auto stage = [&](BwdEvent* buf, const BwdEvent* src, int32_t s0, int32_t end) {
-      if (s0 < end)
-      {
-        const int32_t k = min(s0 + (lane & (BWD_BLK - 1)), end - 1);
-        const double4 v4 = *reinterpret_cast<const double4*>(src + k);
-        *reinterpret_cast<double4*>(buf + (lane & (BWD_BLK - 1))) = v4;
-      }
-      flush();
-    };
 
-    int32_t cs = cb, rs = rb; // first event of the staged blocks
-    stage(bufC, cl, cb, ce);
-    stage(bufR, rl, rb, re);
-    double acc = 0.0;  // integral of H_i from the open record's start (or tlo) to tprev
+    // next chunk event; blocks start at cb + 64 k, so event ic is lane
+    // (ic - cb) & 63 of its block
+    int32_t ic = cb;
+    PartnerBlk Cc = load_partners(cb);
+    __builtin_amdgcn_s_waitcnt(0x0F70); // vmcnt(0): see the block loads below
+    double acc = 0.0;   // integral of H_i from the open record's start (or tlo) to tprev
     double tprev = tlo; // time of the chunk's last event taken
-    int32_t ic = cb, ir = rb;
-    bool hc = ic < ce;
-    BwdEvent C = bufC[0];
-    double Rt = ir < re ? bufR[0].t : __builtin_inf();
-    for (;;)
+
+    // Sum over the chunk's events at time tprev (all taken) of w_ij * jump:
+    // at a breakpoint there, A is the mean of H_i's two limits, H - J / 2.
+    auto jumps_at_tprev = [&](int32_t ic, double tprev) -> double {
+      double J = 0.0;
+      for (int32_t e = ic - 1; e >= cb; --e)
+      {
+        const BwdEvent T = uniform_event(cl, e);
+        if (__double_as_longlong(T.t) != __double_as_longlong(tprev))
+          break;
+        J = fma(static_cast<double>(s_w[(T.col & (CH - 1)) * 64 + lane]), T.d, J);
+      }
+      return J;
+    };
+    // Emission rounds (every lane with a breakpoint before T emits one
+    // record per round) and new row blocks, until no lane is below T.
+    auto drain = [&](double T) {
+      while (__ballot(nb < T))
+      {
+        const bool go = qptr < qend && nb < T;
+        const unsigned long long gom = __ballot(go);
+        if (gom)
+        {
+          double J = 0.0;
+          if (__ballot(go && __double_as_longlong(nb) == __double_as_longlong(tprev)))
+            J = jumps_at_tprev(ic, tprev);
+          const int32_t ngo = __popcll(gom);
+          if (nem + ngo > 64)
+            flush();
+          if (go)
+          {
+            // record npos - 1 ends at nb
+            const double d = nb - tprev;
+            const int32_t slot = nem + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(gom >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(gom), 0u));
+            s_eq[slot] = fma(H, d, acc);
+            s_ea[slot] = fma(-0.5, J, H);
+            s_ep[slot] = npos - 1;
+            acc = -H * d;
+            open = npos;
+            ++qptr;
+            next_own();
+          }
+          nem += ngo;
+        }
+        else
+        {
+          // the lanes below T are done with this block, which ends before T
+          rs += 64;
+          open_block();
+        }
+      }
+    };
+    double Ct = lane_d(Cc.t, 0);
+    while (ic < ce)
     {
-      // the chunk's events up to the band's next breakpoint (chunk first on
-      // equal times, so H is the right limit at a breakpoint)
-      while (hc && C.t <= Rt)
+      // breakpoints before the chunk's event ic see H before it (at equal
+      // times the chunk's events come first)
+      if (__ballot(nb < Ct))
+        drain(Ct);
+      const int l = (ic - cb) & 63;
+      const double Cd = lane_d(Cc.d, l);
+      const int32_t Ccol = __builtin_amdgcn_readlane(Cc.col, l) & (CH - 1);
+      const double w = static_cast<double>(s_w[Ccol * 64 + lane]);
+      acc = fma(H, Ct - tprev, acc);
+      H = fma(w, Cd, H);
+      tprev = Ct;
+      if (++ic < ce)
       {
-        const double w = static_cast<double>(s_w[(C.col & (CH - 1)) * 64 + lane]);
-        acc = fma(H, C.t - tprev, acc);
-        H = fma(w, C.d, H);
-        tprev = C.t;
-        hc = ++ic < ce;
-        if (hc)
+        if (l == 63)
         {
-          if (ic == cs + BWD_BLK)
-          {
-            cs = ic;
-            stage(bufC, cl, cs, ce);
-          }
-          C = bufC[ic - cs];
+          // The next block is loaded when needed and waited for here, once
+          // per block: a block still in flight where the paths of this loop
+          // meet would make the compiler wait on it (and on every atomic
+          // issued before it) at each event. 0x0F70 = vmcnt(0) alone.
+          Cc = load_partners(ic);
+          __builtin_amdgcn_s_waitcnt(0x0F70);
         }
+        Ct = lane_d(Cc.t, (ic - cb) & 63);
       }
-      if (last && !hc)
-        H = Hend;
-      if (ir >= re)
-        break;
-      const BwdEvent R = bufR[ir - rs];
-      if (!(R.col & BWD_FIRST))
-      {
-        // record R.pos - 1 of lane R.col's PCF ends at R.t
-        const double d = R.t - tprev;
-        double J = 0.0;
-        if (ic > cb && __double_as_longlong(R.t) == __double_as_longlong(tprev))
-        {
-          // partners jump at R.t too: A is the mean of H's two limits
-          for (int32_t e = ic - 1; e >= cb; --e)
-          {
-            const BwdEvent T = e >= cs ? bufC[e - cs] : uniform_event(cl, e);
-            if (__double_as_longlong(T.t) != __double_as_longlong(tprev))
-              break;
-            J = fma(static_cast<double>(s_w[(T.col & (CH - 1)) * 64 + lane]), T.d, J);
-          }
-        }
-        if (lane == R.col)
-        {
-          em_q[nem] = fma(H, d, acc);
-          em_a[nem] = fma(-0.5, J, H);
-          em_p[nem] = R.pos - 1;
-          acc = -H * d;
-          open = R.pos;
-        }
-        if (++nem == 64)
-          flush();
-      }
-      if (++ir < re)
-      {
-        if (ir == rs + BWD_BLK)
-        {
-          rs = ir;
-          stage(bufR, rl, rs, re);
-        }
-        Rt = bufR[ir - rs].t;
-      }
-      else
-        Rt = __builtin_inf();
     }
+    // past the chunk's last event H_i is exact
+    if (last)
+      H = Hend;
+    drain(__builtin_inf());
     flush();
     if (act)
     {
