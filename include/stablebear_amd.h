@@ -77,6 +77,25 @@ typedef struct
   int32_t dtype;           /* SBAMD_F32 or SBAMD_F64 */
 } sbamd_pcf_set;
 
+#define SBAMD_HOST 0
+#define SBAMD_DEVICE 1
+
+/* Strided view of a CSR PCF set in host or device memory: point k's time is
+ * at (char*)t + k * t_stride and its value at (char*)v + k * v_stride, so SoA
+ * arrays (stride = element size) and AoS {t, v} points (stride = twice the
+ * element size, v = t + element size) are both views. */
+typedef struct
+{
+  int64_t n;               /* number of PCFs */
+  const int64_t* offsets;  /* n + 1 entries, in `location` memory */
+  const void* t;
+  const void* v;
+  int64_t t_stride;        /* bytes */
+  int64_t v_stride;        /* bytes */
+  int32_t dtype;           /* SBAMD_F32 or SBAMD_F64 */
+  int32_t location;        /* SBAMD_HOST or SBAMD_DEVICE */
+} sbamd_pcf_view;
+
 /* Pair-tile shape of the lane-per-pair kernel: rows x cols pairs per
  * workgroup (rows * cols <= 256), `cap` breakpoint records per PCF staged in
  * LDS per phase. Zero fields mean "use the tuned or default value". */
@@ -122,8 +141,21 @@ int sbamd_ctx_stream(sbamd_ctx* ctx, void** stream);
  * (cuda_pcf_data_manager.cuh:13-23, offset_data_manager.hpp:42-68,
  * cuda_offset_data_manager.cuh:24-44). */
 int sbamd_set_create(sbamd_ctx* ctx, const sbamd_pcf_set* set, int32_t precision, sbamd_set** out);
+/* The same set, bit for bit, validated and packed on the GPU (one wavefront
+ * per PCF): host views are copied to the device once as they lie, device
+ * views are read in place. Same errors and messages as sbamd_set_create.
+ * Replaces the same reference code, which flattens every point on one host
+ * thread (cuda_pcf_data_manager.cuh:13-23, offset_data_manager.hpp:42-68). */
+int sbamd_set_create_gpu(sbamd_ctx* ctx, const sbamd_pcf_view* view, int32_t precision, sbamd_set** out);
 int sbamd_set_destroy(sbamd_set* set);
 int sbamd_set_info(const sbamd_set* set, int64_t* n, int64_t* records, int64_t* points);
+/* Copy the packed device layout to host buffers (any may be NULL): records
+ * (records x *rec_bytes bytes, sorted order), sorted record offsets (n + 1),
+ * sorted position -> PCF (n), record -> source point (records; -1 for the
+ * record of an empty PCF). No reference counterpart: it exposes the device
+ * layout for inspection and tests. */
+int sbamd_set_export(const sbamd_set* set, void* rec, int64_t* roff, int64_t* perm, int64_t* src,
+                     int32_t* rec_bytes);
 /* Replace the values of an existing set from a device array laid out like the
  * original v (offsets[n] entries, set dtype); times and structure unchanged. */
 int sbamd_set_update_values(sbamd_set* set, const void* v_device, void* stream);

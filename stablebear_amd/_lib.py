@@ -47,6 +47,16 @@ class PcfSet(ctypes.Structure):
                 ("dtype", ctypes.c_int32)]
 
 
+HOST = 0
+DEVICE = 1
+
+
+class PcfView(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("offsets", vp), ("t", vp), ("v", vp),
+                ("t_stride", ctypes.c_int64), ("v_stride", ctypes.c_int64),
+                ("dtype", ctypes.c_int32), ("location", ctypes.c_int32)]
+
+
 class TileShape(ctypes.Structure):
     _fields_ = [("rows", ctypes.c_int32), ("cols", ctypes.c_int32), ("cap", ctypes.c_int32),
                 ("reserved", ctypes.c_int32)]
@@ -70,6 +80,9 @@ _SIGS = {
     "sbamd_ctx_synchronize": (None, [vp]),
     "sbamd_ctx_stream": (None, [vp, ctypes.POINTER(vp)]),
     "sbamd_set_create": (None, [vp, ctypes.POINTER(PcfSet), ctypes.c_int32, ctypes.POINTER(vp)]),
+    "sbamd_set_create_gpu": (None, [vp, ctypes.POINTER(PcfView), ctypes.c_int32,
+                                    ctypes.POINTER(vp)]),
+    "sbamd_set_export": (None, [vp, vp, c_i64p, c_i64p, c_i64p, c_i32p]),
     "sbamd_set_destroy": (None, [vp]),
     "sbamd_set_info": (None, [vp, c_i64p, c_i64p, c_i64p]),
     "sbamd_set_update_values": (None, [vp, vp, vp]),

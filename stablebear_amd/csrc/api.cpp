@@ -210,12 +210,160 @@ namespace sbamd
       s.points = pk.points;
       s.h_roff = pk.roff;
       s.h_perm = pk.perm;
-      s.h_src = pk.src;
+      s.has_empty = std::find(pk.src.begin(), pk.src.end(), int64_t(-1)) != pk.src.end();
       use_device(ctx);
       upload(s.roff, pk.roff);
       upload(s.perm, pk.perm);
       upload(s.src, pk.src);
       upload(s.rec, interleave_records(pk, s.tt_f64, s.tv_f64));
+    }
+
+    // The same set as make_set, validated and packed on the device. Host
+    // views are copied once as they lie (one copy when times and values
+    // interleave); device views are read in place.
+    void make_set_gpu(Ctx* ctx, const sbamd_pcf_view& in, int32_t precision, Set& s)
+    {
+      if (in.dtype != SBAMD_F32 && in.dtype != SBAMD_F64)
+        fail(SBAMD_ERR_INVALID, "PCF set dtype must be SBAMD_F32 or SBAMD_F64");
+      if (in.location != SBAMD_HOST && in.location != SBAMD_DEVICE)
+        fail(SBAMD_ERR_INVALID, "PCF view location must be SBAMD_HOST or SBAMD_DEVICE");
+      if (in.n < 0)
+        fail(SBAMD_ERR_INVALID, "PCF set size must be non-negative");
+      if (in.n > std::numeric_limits<int32_t>::max() / 2)
+        fail(SBAMD_ERR_INVALID, "PCF set too large (more than 2^30 PCFs)");
+      if (in.n > 0 && !in.offsets)
+        fail(SBAMD_ERR_INVALID, "PCF set offsets are NULL");
+      const int64_t es = in.dtype == SBAMD_F64 ? 8 : 4;
+      use_device(ctx);
+      const hipStream_t st = ctx->stream;
+
+      std::vector<int64_t> h_off(static_cast<size_t>(in.n) + 1, 0);
+      if (in.n > 0)
+      {
+        if (in.location == SBAMD_HOST)
+          std::memcpy(h_off.data(), in.offsets, h_off.size() * sizeof(int64_t));
+        else
+          SBAMD_HIP(hipMemcpy(h_off.data(), in.offsets, h_off.size() * sizeof(int64_t),
+                              hipMemcpyDeviceToHost));
+      }
+      // offsets: the host packer's rules
+      check_set_offsets(in.n, h_off.data(), in.dtype);
+      const int64_t M = in.n > 0 ? h_off[in.n] : 0;
+      if (M > 0 && (!in.t || !in.v))
+        fail(SBAMD_ERR_INVALID, "PCF set time/value arrays are NULL");
+      if (M > 0 && (in.t_stride < es || in.v_stride < es))
+        fail(SBAMD_ERR_INVALID, "PCF view strides must be at least the element size");
+
+      s.ctx = ctx;
+      s.precision = precision;
+      s.in_dtype = in.dtype;
+      set_layout_for(precision, in.dtype, s.tt_f64, s.tv_f64);
+      s.n = in.n;
+      s.points = M;
+      s.has_empty = false;
+      for (int64_t i = 0; i < in.n; ++i)
+        s.has_empty |= h_off[i + 1] == h_off[i];
+
+      DeviceBuffer d_off, d_pts, d_t, d_v;
+      upload(d_off, h_off);
+      const unsigned char* tp = static_cast<const unsigned char*>(in.t);
+      const unsigned char* vp = static_cast<const unsigned char*>(in.v);
+      if (M > 0 && in.location == SBAMD_HOST)
+      {
+        // byte spans of the times and the values; one copy of their union
+        // when they overlap (AoS points), else one copy each
+        const unsigned char* t_end = tp + (M - 1) * in.t_stride + es;
+        const unsigned char* v_end = vp + (M - 1) * in.v_stride + es;
+        if (tp < v_end && vp < t_end)
+        {
+          const unsigned char* lo = std::min(tp, vp);
+          const unsigned char* hi = std::max(t_end, v_end);
+          d_pts = DeviceBuffer(static_cast<size_t>(hi - lo));
+          SBAMD_HIP(hipMemcpy(d_pts.ptr, lo, d_pts.bytes, hipMemcpyHostToDevice));
+          tp = d_pts.as<unsigned char>() + (tp - lo);
+          vp = d_pts.as<unsigned char>() + (vp - lo);
+        }
+        else
+        {
+          d_t = DeviceBuffer(static_cast<size_t>(t_end - tp));
+          d_v = DeviceBuffer(static_cast<size_t>(v_end - vp));
+          SBAMD_HIP(hipMemcpy(d_t.ptr, tp, d_t.bytes, hipMemcpyHostToDevice));
+          SBAMD_HIP(hipMemcpy(d_v.ptr, vp, d_v.bytes, hipMemcpyHostToDevice));
+          tp = d_t.as<unsigned char>();
+          vp = d_v.as<unsigned char>();
+        }
+      }
+
+      PackParams P;
+      std::memset(&P, 0, sizeof(P));
+      P.n = in.n;
+      P.off = d_off.as<int64_t>();
+      P.t = tp;
+      P.v = vp;
+      P.t_stride = in.t_stride;
+      P.v_stride = in.v_stride;
+      P.tt_f64 = s.tt_f64;
+      P.tv_f64 = s.tv_f64;
+      DeviceBuffer d_m(static_cast<size_t>(in.n) * sizeof(int64_t));
+      DeviceBuffer d_bad(sizeof(unsigned long long));
+      const unsigned long long none = ~0ull;
+      SBAMD_HIP(hipMemcpy(d_bad.ptr, &none, sizeof(none), hipMemcpyHostToDevice));
+      P.m = d_m.as<int64_t>();
+      P.bad = d_bad.as<unsigned long long>();
+      launch_pack_count(P, in.dtype == SBAMD_F64, st);
+      unsigned long long bad = none;
+      std::vector<int64_t> m(static_cast<size_t>(in.n));
+      SBAMD_HIP(hipMemcpyAsync(&bad, d_bad.ptr, sizeof(bad), hipMemcpyDeviceToHost, st));
+      if (in.n > 0)
+        SBAMD_HIP(hipMemcpyAsync(m.data(), d_m.ptr, m.size() * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+      SBAMD_HIP(hipStreamSynchronize(st));
+      if (bad != none)
+      {
+        // the host rules on the first offending PCF give the message
+        const int64_t i = static_cast<int64_t>(bad);
+        const int64_t b = h_off[i], cnt = h_off[i + 1] - b;
+        std::vector<unsigned char> raw_t(static_cast<size_t>(cnt * es)), raw_v(raw_t.size());
+        for (int64_t k = 0; k < cnt; ++k)
+        {
+          const hipMemcpyKind kind = hipMemcpyDeviceToHost;
+          SBAMD_HIP(hipMemcpy(raw_t.data() + k * es, tp + (b + k) * in.t_stride, es, kind));
+          SBAMD_HIP(hipMemcpy(raw_v.data() + k * es, vp + (b + k) * in.v_stride, es, kind));
+        }
+        const int64_t one_off[2] = {0, cnt};
+        sbamd_pcf_set one;
+        one.n = 1;
+        one.offsets = one_off;
+        one.t = raw_t.data();
+        one.v = raw_v.data();
+        one.dtype = in.dtype;
+        validate_set(one);
+        fail(SBAMD_ERR_RUNTIME, "device validation flagged PCF " + std::to_string(i) +
+                                    " but the host rules accept it");
+      }
+
+      std::vector<int64_t> perm;
+      sort_by_records(m, perm);
+      std::vector<int64_t> roff(static_cast<size_t>(in.n) + 1, 0), qinv(static_cast<size_t>(in.n));
+      for (int64_t q = 0; q < in.n; ++q)
+      {
+        roff[q + 1] = roff[q] + m[perm[q]];
+        qinv[perm[q]] = q;
+      }
+      s.records = roff.back();
+      s.h_roff = roff;
+      s.h_perm = perm;
+      upload(s.roff, roff);
+      upload(s.perm, perm);
+      DeviceBuffer d_qinv;
+      upload(d_qinv, qinv);
+      s.rec = DeviceBuffer(static_cast<size_t>(s.records) * record_bytes(s.tt_f64));
+      s.src = DeviceBuffer(static_cast<size_t>(s.records) * sizeof(int64_t));
+      P.roff = s.roff.as<int64_t>();
+      P.qinv = d_qinv.as<int64_t>();
+      P.rec = s.rec.ptr;
+      P.src = s.src.as<int64_t>();
+      launch_pack_records(P, in.dtype == SBAMD_F64, st);
+      SBAMD_HIP(hipStreamSynchronize(st));
     }
 
     std::vector<int64_t> sorted_counts(const Set& s)
@@ -529,6 +677,19 @@ extern "C"
     });
   }
 
+  int sbamd_set_create_gpu(sbamd_ctx* ctx, const sbamd_pcf_view* view, int32_t precision, sbamd_set** out)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      need(view, "view");
+      need(out, "out");
+      *out = nullptr;
+      auto s = std::make_unique<sbamd_set>();
+      make_set_gpu(ctx, *view, precision, *s);
+      *out = s.release();
+    });
+  }
+
   int sbamd_set_destroy(sbamd_set* set)
   {
     return guarded([&] {
@@ -552,6 +713,26 @@ extern "C"
     });
   }
 
+  int sbamd_set_export(const sbamd_set* set, void* rec, int64_t* roff, int64_t* perm, int64_t* src,
+                       int32_t* rec_bytes)
+  {
+    return guarded([&] {
+      need(set, "set");
+      use_device(set->ctx);
+      SBAMD_HIP(hipDeviceSynchronize());
+      auto get = [](void* dst, const DeviceBuffer& b) {
+        if (dst && b.bytes)
+          SBAMD_HIP(hipMemcpy(dst, b.ptr, b.bytes, hipMemcpyDeviceToHost));
+      };
+      get(rec, set->rec);
+      get(roff, set->roff);
+      get(perm, set->perm);
+      get(src, set->src);
+      if (rec_bytes)
+        *rec_bytes = record_bytes(set->tt_f64);
+    });
+  }
+
   int sbamd_set_update_values(sbamd_set* set, const void* v_device, void* stream)
   {
     return guarded([&] {
@@ -560,9 +741,8 @@ extern "C"
       use_device(set->ctx);
       if (set->records > 0 && set->points == 0)
         return; // only zero PCFs: nothing to refresh
-      for (int64_t r : set->h_src)
-        if (r < 0)
-          fail(SBAMD_ERR_INVALID, "value updates need sets without empty PCFs");
+      if (set->has_empty)
+        fail(SBAMD_ERR_INVALID, "value updates need sets without empty PCFs");
       launch_update_values(v_device, set->src.as<int64_t>(), set->records, set->rec.ptr,
                            record_bytes(set->tt_f64), set->in_dtype == SBAMD_F64, set->tv_f64,
                            pick_stream(set->ctx, stream));

@@ -34,24 +34,29 @@ namespace sbamd
 
     void check_header(const sbamd_pcf_set& s)
     {
-      if (s.dtype != SBAMD_F32 && s.dtype != SBAMD_F64)
-        fail(SBAMD_ERR_INVALID, "PCF set dtype must be SBAMD_F32 or SBAMD_F64");
-      if (s.n < 0)
-        fail(SBAMD_ERR_INVALID, "PCF set size must be non-negative");
-      if (s.n > std::numeric_limits<int32_t>::max() / 2)
-        fail(SBAMD_ERR_INVALID, "PCF set too large (more than 2^30 PCFs)");
-      if (s.n > 0 && !s.offsets)
-        fail(SBAMD_ERR_INVALID, "PCF set offsets are NULL");
-      if (s.n == 0)
-        return;
-      if (s.offsets[0] != 0)
-        fail(SBAMD_ERR_INVALID, "PCF set offsets must start at 0");
-      for (int64_t i = 0; i < s.n; ++i)
-        if (s.offsets[i + 1] < s.offsets[i])
-          fail(SBAMD_ERR_INVALID, "PCF set offsets must be non-decreasing");
-      if (s.offsets[s.n] > 0 && (!s.t || !s.v))
+      check_set_offsets(s.n, s.offsets, s.dtype);
+      if (s.n > 0 && s.offsets[s.n] > 0 && (!s.t || !s.v))
         fail(SBAMD_ERR_INVALID, "PCF set time/value arrays are NULL");
     }
+  }
+
+  void check_set_offsets(int64_t n, const int64_t* offsets, int32_t dtype)
+  {
+    if (dtype != SBAMD_F32 && dtype != SBAMD_F64)
+      fail(SBAMD_ERR_INVALID, "PCF set dtype must be SBAMD_F32 or SBAMD_F64");
+    if (n < 0)
+      fail(SBAMD_ERR_INVALID, "PCF set size must be non-negative");
+    if (n > std::numeric_limits<int32_t>::max() / 2)
+      fail(SBAMD_ERR_INVALID, "PCF set too large (more than 2^30 PCFs)");
+    if (n > 0 && !offsets)
+      fail(SBAMD_ERR_INVALID, "PCF set offsets are NULL");
+    if (n == 0)
+      return;
+    if (offsets[0] != 0)
+      fail(SBAMD_ERR_INVALID, "PCF set offsets must start at 0");
+    for (int64_t i = 0; i < n; ++i)
+      if (offsets[i + 1] < offsets[i])
+        fail(SBAMD_ERR_INVALID, "PCF set offsets must be non-decreasing");
   }
 
   // Breakpoint rules of the reference's Pcf constructor

@@ -202,7 +202,9 @@ namespace sbamd
 
   void pack_set(const sbamd_pcf_set& in, PackedSet& out);
   void validate_set(const sbamd_pcf_set& in);
+  void check_set_offsets(int64_t n, const int64_t* offsets, int32_t dtype);
   std::vector<int64_t> record_counts(const sbamd_pcf_set& in);
+  void sort_by_records(const std::vector<int64_t>& m, std::vector<int64_t>& perm);
 
   struct Ctx
   {
@@ -224,11 +226,10 @@ namespace sbamd
     int64_t points = 0;
     std::vector<int64_t> h_roff; // sorted record offsets
     std::vector<int64_t> h_perm;
-    std::vector<int64_t> h_src;
+    bool has_empty = false; // some PCF has no points (its record has no source point)
     // rec: one record per constant piece, {t_end, value} in the set's time and
     // value types (16 bytes with FP64 times, 8 with FP32 times), sorted order
     DeviceBuffer roff, rec, perm, src;
-    DeviceBuffer inv_src; // original point -> record (or -1), for value updates
   };
 
   struct Plan
@@ -273,6 +274,25 @@ namespace sbamd
   void launch_norm(const NormParams& prm, int tt_f64, int tv_f64, hipStream_t stream);
   void launch_backward(const BwdParams& prm, hipStream_t stream);
   void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream);
+
+  // Device-side packing of a strided CSR view (kernels_pack.hip).
+  struct PackParams
+  {
+    int64_t n;                 // PCFs
+    const int64_t* off;        // n + 1 point offsets (device)
+    const unsigned char* t;    // point k's time at t + k * t_stride
+    const unsigned char* v;    // point k's value at v + k * v_stride
+    int64_t t_stride, v_stride;
+    int64_t* m;                // pass 1: records per PCF (original order)
+    unsigned long long* bad;   // pass 1: lowest PCF index breaking a rule
+    const int64_t* roff;       // pass 2: sorted record offsets (n + 1)
+    const int64_t* qinv;       // pass 2: PCF -> sorted position
+    void* rec;                 // pass 2: records
+    int64_t* src;              // pass 2: record -> point (-1: zero PCF)
+    int32_t tt_f64, tv_f64;    // record types
+  };
+  void launch_pack_count(const PackParams& P, int f64, hipStream_t stream);
+  void launch_pack_records(const PackParams& P, int f64, hipStream_t stream);
   void launch_rectangles(const double* ft, const double* fv, int64_t mf, const double* gt,
                          const double* gv, int64_t mg, double a, double b, double* out,
                          int64_t cap, int64_t* count, hipStream_t stream);

@@ -66,6 +66,20 @@ class Csr:
         return s
 
 
+def csr_view(csr: Csr) -> L.PcfView:
+    """Host view of a CSR set for the GPU packer."""
+    view = L.PcfView()
+    view.n = csr.n
+    view.offsets = csr.offsets.ctypes.data
+    view.t = csr.t.ctypes.data if csr.t.size else None
+    view.v = csr.v.ctypes.data if csr.v.size else None
+    view.t_stride = csr.t.strides[0] if csr.t.ndim else csr.t.itemsize
+    view.v_stride = csr.v.strides[0] if csr.v.ndim else csr.v.itemsize
+    view.dtype = csr.dtype
+    view.location = L.HOST
+    return view
+
+
 def csr_from_arrays(offsets, t, v) -> Csr:
     """CSR set from offsets (n+1), times and values (float32 or float64)."""
     t = np.asarray(t)
@@ -250,22 +264,78 @@ PRECISIONS = {"fp64": L.PREC_FP64, "fp32c": L.PREC_FP32C}
 
 
 class DeviceSet:
-    """A PCF set validated, sorted and uploaded once for a given precision."""
+    """A PCF set validated, sorted and uploaded once for a given precision.
 
-    def __init__(self, csr: Csr, precision: str = "fp64", device: int | None = None):
+    ``pack="host"`` packs the records on the host and uploads them;
+    ``pack="gpu"`` copies the CSR arrays to the device as they are and
+    validates and packs there (the same records, bit for bit)."""
+
+    def __init__(self, csr: Csr | None, precision: str = "fp64", device: int | None = None,
+                 pack: str = "host", view: L.PcfView | None = None, keep=None):
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        if pack not in ("host", "gpu"):
+            raise ValueError("pack must be 'host' or 'gpu'")
         self.ctx = context(device)
         self.csr = csr
         self.precision = precision
-        s = csr.struct()
+        self._keep = keep
         h = ctypes.c_void_p()
-        L.check(L.lib().sbamd_set_create(self.ctx, ctypes.byref(s), PRECISIONS[precision],
-                                         ctypes.byref(h)))
+        if view is not None:
+            L.check(L.lib().sbamd_set_create_gpu(self.ctx, ctypes.byref(view),
+                                                 PRECISIONS[precision], ctypes.byref(h)))
+        elif pack == "gpu":
+            view = csr_view(csr)
+            L.check(L.lib().sbamd_set_create_gpu(self.ctx, ctypes.byref(view),
+                                                 PRECISIONS[precision], ctypes.byref(h)))
+        else:
+            s = csr.struct()
+            L.check(L.lib().sbamd_set_create(self.ctx, ctypes.byref(s), PRECISIONS[precision],
+                                             ctypes.byref(h)))
         self.handle = h
         n, rec, pts = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         L.check(L.lib().sbamd_set_info(h, ctypes.byref(n), ctypes.byref(rec), ctypes.byref(pts)))
         self.n, self.records, self.points = n.value, rec.value, pts.value
+
+    @classmethod
+    def from_device(cls, offsets, t, v, precision: str = "fp64", device: int | None = None):
+        """Set from CSR arrays already in device memory (torch tensors on the
+        GPU: offsets int64 (n+1), t and v float32/float64 of one dtype, any
+        element strides); validated and packed on the GPU, read in place."""
+        import torch
+
+        if offsets.dtype != torch.int64 or offsets.dim() != 1 or not offsets.is_contiguous():
+            raise ValueError("offsets must be a contiguous 1-D int64 tensor")
+        if t.dtype != v.dtype or t.dtype not in (torch.float32, torch.float64):
+            raise TypeError("t and v must both be float32 or both float64")
+        if t.dim() != 1 or v.dim() != 1 or t.numel() != v.numel():
+            raise ValueError("t and v must be 1-D tensors of equal length")
+        es = t.element_size()
+        view = L.PcfView()
+        view.n = offsets.numel() - 1
+        view.offsets = offsets.data_ptr()
+        view.t = t.data_ptr() if t.numel() else None
+        view.v = v.data_ptr() if v.numel() else None
+        view.t_stride = t.stride(0) * es
+        view.v_stride = v.stride(0) * es
+        view.dtype = L.F64 if t.dtype == torch.float64 else L.F32
+        view.location = L.DEVICE
+        return cls(None, precision, device, view=view, keep=(offsets, t, v))
+
+    def export(self):
+        """The packed device layout as numpy arrays: (records as raw bytes,
+        sorted record offsets, permutation, source points)."""
+        rb = ctypes.c_int32()
+        L.check(L.lib().sbamd_set_export(self.handle, None, None, None, None, ctypes.byref(rb)))
+        rec = np.zeros(self.records * rb.value, dtype=np.uint8)
+        roff = np.zeros(self.n + 1, dtype=np.int64)
+        perm = np.zeros(self.n, dtype=np.int64)
+        src = np.zeros(self.records, dtype=np.int64)
+        L.check(L.lib().sbamd_set_export(
+            self.handle, rec.ctypes.data if rec.size else None, roff.ctypes.data_as(L.c_i64p),
+            perm.ctypes.data_as(L.c_i64p) if perm.size else None,
+            src.ctypes.data_as(L.c_i64p) if src.size else None, ctypes.byref(rb)))
+        return rec, roff, perm, src
 
     def update_values(self, v_device_ptr: int, stream: int | None = None) -> None:
         L.check(L.lib().sbamd_set_update_values(self.handle, v_device_ptr, stream))
