@@ -235,6 +235,11 @@ def main(argv=None):
     ap.add_argument("--cdist-n", type=int, default=50000)
     ap.add_argument("--cdist-rows", type=int, default=48,
                     help="rows of the cdist leg checked against the reference walk on the CPU")
+    ap.add_argument("--ingest", action="store_true",
+                    help="run only the ingest leg: one side of the cdist workload (N = 50,000, "
+                         "m ~ U{500..5000}, FP64) packed into a device set on the host and on "
+                         "the GPU")
+    ap.add_argument("--ingest-n", type=int, default=50000)
     ap.add_argument("--long", action="store_true",
                     help="run only the long-pair leg: L1 pdist of 64 PCFs with m ~ U{50k..100k}, "
                          "tile kernel vs long-pair kernel")
@@ -269,6 +274,9 @@ def main(argv=None):
         return 0
     if args.long:
         print(json.dumps(_long_leg(args, stream)), flush=True)
+        return 0
+    if args.ingest:
+        print(json.dumps(_ingest_leg(args)), flush=True)
         return 0
 
     n = args.n
@@ -494,6 +502,60 @@ def _batched_leg(args, stream):
                          "sample": f"the {checked} smallest problems ({cpu_pairs} pairs, "
                                    f"{cpu_s:.1f} s)"},
     }
+
+
+def _ingest_leg(args):
+    """Device-set ingest of one side of the cdist workload (N PCFs, m ~ U{500..5000},
+    FP64 times and values, FP32C set): the host packer (validate, drop zero-length
+    pieces, sort, interleave on one host thread, then one upload of the records)
+    against the GPU packer fed from host arrays (one upload of t and v as they lie)
+    and from arrays already in HBM. Wall-clock seconds per set, best of `reps`; the
+    packed layouts are compared byte for byte."""
+    import torch
+
+    from stablebear_amd import runtime, workloads
+
+    t0 = time.perf_counter()
+    off, t, v = workloads.c5_side(args.ingest_n)
+    csr = runtime.csr_from_arrays(off, t, v)
+    M = int(off[-1])
+    _progress(f"ingest: {args.ingest_n} PCFs, {M} points generated in "
+              f"{time.perf_counter() - t0:.1f} s")
+
+    def wall(make, reps):
+        best = None
+        keep = None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ds = make()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+            keep = ds
+        return best, keep
+
+    host_s, host_set = wall(lambda: runtime.DeviceSet(csr, "fp32c"), 2)
+    _progress(f"ingest: host packer {host_s:.3f} s")
+    gh_s, gh_set = wall(lambda: runtime.DeviceSet(csr, "fp32c", pack="gpu"), 3)
+    _progress(f"ingest: GPU packer from host arrays {gh_s:.3f} s")
+    d_off = torch.from_numpy(off).cuda()
+    d_t = torch.from_numpy(t).cuda()
+    d_v = torch.from_numpy(v).cuda()
+    gd_s, gd_set = wall(lambda: runtime.DeviceSet.from_device(d_off, d_t, d_v, "fp32c"), 3)
+    _progress(f"ingest: GPU packer from device arrays {gd_s:.3f} s")
+    a = host_set.export()
+    identical = all(np.array_equal(x, y) for x, y in zip(a, gh_set.export()))
+    identical = identical and all(np.array_equal(x, y) for x, y in zip(a, gd_set.export()))
+    in_bytes = M * 16
+    return {"ingest": {
+        "workload": f"one side of L1 cdist: N={args.ingest_n}, m~U{{500..5000}}, FP64 t and v, "
+                    "FP32C set",
+        "points": M, "records": host_set.records, "input_bytes": in_bytes,
+        "host_pack_s": host_s, "gpu_pack_from_host_s": gh_s, "gpu_pack_from_device_s": gd_s,
+        "host_over_gpu_from_host": host_s / gh_s, "host_over_gpu_from_device": host_s / gd_s,
+        "gpu_from_device_input_GBs": in_bytes / gd_s / 1e9,
+        "layouts_identical": bool(identical)}}
 
 
 def _cdist_leg(args, stream):

@@ -86,15 +86,17 @@ def c4(n=4096, seed=4004):
     return off, t, v, w
 
 
+def c5_side(n=50000, seed=5005, stream=0, m_lo=500, m_hi=5000):
+    """One side (X: stream 0, Y: stream 1) of c5 as a CSR triple."""
+    rng = np.random.default_rng([seed, stream])
+    m = rng.integers(m_lo, m_hi + 1, size=n).astype(np.int64)
+    off, t = _times(rng, m, 10.0)
+    return off, t, _normal_values(rng, off)
+
+
 def c5(n=50000, seed=5005, m_lo=500, m_hi=5000):
     """(X, Y) CSR triples."""
-    out = []
-    for stream in (0, 1):
-        rng = np.random.default_rng([seed, stream])
-        m = rng.integers(m_lo, m_hi + 1, size=n).astype(np.int64)
-        off, t = _times(rng, m, 10.0)
-        out.append((off, t, _normal_values(rng, off)))
-    return out[0], out[1]
+    return (c5_side(n, seed, 0, m_lo, m_hi), c5_side(n, seed, 1, m_lo, m_hi))
 
 
 def to_pcfs(off, t, v, dtype=np.float64):
