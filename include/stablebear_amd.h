@@ -282,6 +282,54 @@ int sbamd_iterate_rectangles(sbamd_ctx* ctx, const sbamd_pcf_set* f, const sbamd
                              double a, double b, void* out, int64_t cap, int64_t* count);
 
 /* ---- host-only planning helpers (no device needed) ---- */
+/* ---- the reference's binary file format (include/sbear/io.hpp:27-158,
+ * io/tensor_io.hpp, io/pcf_io.hpp, io/compressed_matrix_io.hpp) ----
+ * Files hold one tensor or one object behind a header ("\1MPCF", 'e', format
+ * version, three length-prefixed strings). These functions work on whole
+ * file images in memory; format errors return SBAMD_ERR_RUNTIME with the
+ * reference's messages. */
+#define SBAMD_IO_TENSOR 1
+#define SBAMD_IO_OBJECT 2
+typedef struct
+{
+  int32_t version;      /* format version of the file (1 or 2) */
+  uint32_t kind;        /* SBAMD_IO_TENSOR or SBAMD_IO_OBJECT */
+  int32_t base_format;  /* element type (tensor_io.hpp:92-121): 1 float, 2 int, 3 uint,
+                           4 bool, 100 PCF, 101 integer PCF, 1000 point cloud,
+                           1100 symmetric matrix, 1120 distance matrix, 10000 barcode */
+  int32_t sub_format;   /* element width in bits */
+  int32_t ndim;         /* tensor rank (0 for objects) */
+  int32_t reserved;
+  uint64_t shape[32];
+  uint64_t elements;    /* elements in row-major order (a rank-0 tensor has none; an object one) */
+  uint64_t payload;     /* byte offset of the first element */
+} sbamd_io_info;
+/* Parse the header, format and shape of a file image
+ * (replaces sb::read_any_tensor / read_any_object, io.hpp:222-300). */
+int sbamd_io_parse(const void* buf, uint64_t len, sbamd_io_info* info);
+/* Byte extent of every element: for PCFs counts[e] = points and starts[e] =
+ * byte offset of the first (time, value) pair; for compressed matrices
+ * counts[e] = n and starts[e] = offset of the first stored value; numbers
+ * are contiguous (counts 1). *end = offset past the last element. Checks
+ * that everything lies inside the image (replaces read_tensor,
+ * tensor_io.hpp:213-241). Point clouds and barcodes: SBAMD_ERR_INVALID. */
+int sbamd_io_index(const void* buf, uint64_t len, const sbamd_io_info* info, uint64_t* counts,
+                   uint64_t* starts, uint64_t* end);
+/* Header, kind, format and (for tensors) shape of a file holding one tensor
+ * or object; *size = bytes (out may be NULL to size). Replaces write_header +
+ * write_contiguous_tensor's prologue (io.hpp:119-127, tensor_io.hpp:166-177). */
+int sbamd_io_header(uint32_t kind, int32_t base_format, int32_t sub_format, int32_t ndim,
+                    const uint64_t* shape, void* out, uint64_t cap, uint64_t* size);
+/* PCFs of a CSR set as file elements, [u64 points][(t, v) x points] each, in
+ * order; *size = bytes (out may be NULL to size). Replaces the PCF element
+ * writer (pcf_io.hpp:10-14, point_io.hpp:10-15). */
+int sbamd_io_write_pcfs(int64_t n, const int64_t* offsets, const void* t, const void* v, int32_t dtype,
+                        void* out, uint64_t cap, uint64_t* size);
+/* Device set straight from a file image holding a tensor (any shape,
+ * row-major order) or an object of pcf32 / pcf64: the PCF payload is copied
+ * to the device once as it lies and validated and packed on the GPU. */
+int sbamd_set_create_image(sbamd_ctx* ctx, const void* buf, uint64_t len, int32_t precision, sbamd_set** out);
+
 /* Validate a host set like sbamd_set_create would; returns the status. */
 int sbamd_validate_set(const sbamd_pcf_set* set);
 /* Count union segments of the walk over all pairs of a pdist (cols == NULL,

@@ -57,6 +57,18 @@ class PcfView(ctypes.Structure):
                 ("dtype", ctypes.c_int32), ("location", ctypes.c_int32)]
 
 
+IO_TENSOR = 1
+IO_OBJECT = 2
+
+
+class IoInfo(ctypes.Structure):
+    _fields_ = [("version", ctypes.c_int32), ("kind", ctypes.c_uint32),
+                ("base_format", ctypes.c_int32), ("sub_format", ctypes.c_int32),
+                ("ndim", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("shape", ctypes.c_uint64 * 32), ("elements", ctypes.c_uint64),
+                ("payload", ctypes.c_uint64)]
+
+
 class TileShape(ctypes.Structure):
     _fields_ = [("rows", ctypes.c_int32), ("cols", ctypes.c_int32), ("cap", ctypes.c_int32),
                 ("reserved", ctypes.c_int32)]
@@ -83,6 +95,13 @@ _SIGS = {
     "sbamd_set_create_gpu": (None, [vp, ctypes.POINTER(PcfView), ctypes.c_int32,
                                     ctypes.POINTER(vp)]),
     "sbamd_set_export": (None, [vp, vp, c_i64p, c_i64p, c_i64p, c_i32p]),
+    "sbamd_set_create_image": (None, [vp, vp, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(vp)]),
+    "sbamd_io_parse": (None, [vp, ctypes.c_uint64, ctypes.POINTER(IoInfo)]),
+    "sbamd_io_index": (None, [vp, ctypes.c_uint64, ctypes.POINTER(IoInfo), c_u64p, c_u64p, c_u64p]),
+    "sbamd_io_header": (None, [ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                               c_u64p, vp, ctypes.c_uint64, c_u64p]),
+    "sbamd_io_write_pcfs": (None, [ctypes.c_int64, c_i64p, vp, vp, ctypes.c_int32, vp,
+                                   ctypes.c_uint64, c_u64p]),
     "sbamd_set_destroy": (None, [vp]),
     "sbamd_set_info": (None, [vp, c_i64p, c_i64p, c_i64p]),
     "sbamd_set_update_values": (None, [vp, vp, vp]),

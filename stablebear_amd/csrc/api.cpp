@@ -218,6 +218,95 @@ namespace sbamd
       upload(s.rec, interleave_records(pk, s.tt_f64, s.tv_f64));
     }
 
+    // Device packing of points already on the device (tp / vp, strides in
+    // bytes; PCF i's points offset by pbase[i] bytes when given), for a set
+    // whose type fields are filled in. The lowest PCF breaking a rule is
+    // re-validated on the host for the host packer's exact message.
+    void pack_on_gpu(Ctx* ctx, Set& s, const std::vector<int64_t>& h_off, const unsigned char* tp,
+                     const unsigned char* vp, int64_t t_stride, int64_t v_stride, int32_t dtype,
+                     const std::vector<int64_t>* pbase)
+    {
+      const int64_t n = s.n;
+      const int64_t es = dtype == SBAMD_F64 ? 8 : 4;
+      const hipStream_t st = ctx->stream;
+      DeviceBuffer d_off, d_pbase;
+      upload(d_off, h_off);
+      if (pbase)
+        upload(d_pbase, *pbase);
+      PackParams P;
+      std::memset(&P, 0, sizeof(P));
+      P.n = n;
+      P.off = d_off.as<int64_t>();
+      P.pbase = pbase ? d_pbase.as<int64_t>() : nullptr;
+      P.t = tp;
+      P.v = vp;
+      P.t_stride = t_stride;
+      P.v_stride = v_stride;
+      P.tt_f64 = s.tt_f64;
+      P.tv_f64 = s.tv_f64;
+      DeviceBuffer d_m(static_cast<size_t>(n) * sizeof(int64_t));
+      DeviceBuffer d_bad(sizeof(unsigned long long));
+      const unsigned long long none = ~0ull;
+      SBAMD_HIP(hipMemcpy(d_bad.ptr, &none, sizeof(none), hipMemcpyHostToDevice));
+      P.m = d_m.as<int64_t>();
+      P.bad = d_bad.as<unsigned long long>();
+      launch_pack_count(P, dtype == SBAMD_F64, st);
+      unsigned long long bad = none;
+      std::vector<int64_t> m(static_cast<size_t>(n));
+      SBAMD_HIP(hipMemcpyAsync(&bad, d_bad.ptr, sizeof(bad), hipMemcpyDeviceToHost, st));
+      if (n > 0)
+        SBAMD_HIP(hipMemcpyAsync(m.data(), d_m.ptr, m.size() * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+      SBAMD_HIP(hipStreamSynchronize(st));
+      if (bad != none)
+      {
+        // the host rules on the first offending PCF give the message
+        const int64_t i = static_cast<int64_t>(bad);
+        const int64_t b = h_off[i], cnt = h_off[i + 1] - b;
+        const int64_t pb = pbase ? (*pbase)[i] : 0;
+        std::vector<unsigned char> raw_t(static_cast<size_t>(cnt * es)), raw_v(raw_t.size());
+        for (int64_t k = 0; k < cnt; ++k)
+        {
+          const hipMemcpyKind kind = hipMemcpyDeviceToHost;
+          SBAMD_HIP(hipMemcpy(raw_t.data() + k * es, tp + pb + (b + k) * t_stride, es, kind));
+          SBAMD_HIP(hipMemcpy(raw_v.data() + k * es, vp + pb + (b + k) * v_stride, es, kind));
+        }
+        const int64_t one_off[2] = {0, cnt};
+        sbamd_pcf_set one;
+        one.n = 1;
+        one.offsets = one_off;
+        one.t = raw_t.data();
+        one.v = raw_v.data();
+        one.dtype = dtype;
+        validate_set(one);
+        fail(SBAMD_ERR_RUNTIME, "device validation flagged PCF " + std::to_string(i) +
+                                    " but the host rules accept it");
+      }
+
+      std::vector<int64_t> perm;
+      sort_by_records(m, perm);
+      std::vector<int64_t> roff(static_cast<size_t>(n) + 1, 0), qinv(static_cast<size_t>(n));
+      for (int64_t q = 0; q < n; ++q)
+      {
+        roff[q + 1] = roff[q] + m[perm[q]];
+        qinv[perm[q]] = q;
+      }
+      s.records = roff.back();
+      s.h_roff = roff;
+      s.h_perm = perm;
+      upload(s.roff, roff);
+      upload(s.perm, perm);
+      DeviceBuffer d_qinv;
+      upload(d_qinv, qinv);
+      s.rec = DeviceBuffer(static_cast<size_t>(s.records) * record_bytes(s.tt_f64));
+      s.src = DeviceBuffer(static_cast<size_t>(s.records) * sizeof(int64_t));
+      P.roff = s.roff.as<int64_t>();
+      P.qinv = d_qinv.as<int64_t>();
+      P.rec = s.rec.ptr;
+      P.src = s.src.as<int64_t>();
+      launch_pack_records(P, dtype == SBAMD_F64, st);
+      SBAMD_HIP(hipStreamSynchronize(st));
+    }
+
     // The same set as make_set, validated and packed on the device. Host
     // views are copied once as they lie (one copy when times and values
     // interleave); device views are read in place.
@@ -235,7 +324,6 @@ namespace sbamd
         fail(SBAMD_ERR_INVALID, "PCF set offsets are NULL");
       const int64_t es = in.dtype == SBAMD_F64 ? 8 : 4;
       use_device(ctx);
-      const hipStream_t st = ctx->stream;
 
       std::vector<int64_t> h_off(static_cast<size_t>(in.n) + 1, 0);
       if (in.n > 0)
@@ -264,8 +352,7 @@ namespace sbamd
       for (int64_t i = 0; i < in.n; ++i)
         s.has_empty |= h_off[i + 1] == h_off[i];
 
-      DeviceBuffer d_off, d_pts, d_t, d_v;
-      upload(d_off, h_off);
+      DeviceBuffer d_pts, d_t, d_v;
       const unsigned char* tp = static_cast<const unsigned char*>(in.t);
       const unsigned char* vp = static_cast<const unsigned char*>(in.v);
       if (M > 0 && in.location == SBAMD_HOST)
@@ -294,76 +381,71 @@ namespace sbamd
         }
       }
 
-      PackParams P;
-      std::memset(&P, 0, sizeof(P));
-      P.n = in.n;
-      P.off = d_off.as<int64_t>();
-      P.t = tp;
-      P.v = vp;
-      P.t_stride = in.t_stride;
-      P.v_stride = in.v_stride;
-      P.tt_f64 = s.tt_f64;
-      P.tv_f64 = s.tv_f64;
-      DeviceBuffer d_m(static_cast<size_t>(in.n) * sizeof(int64_t));
-      DeviceBuffer d_bad(sizeof(unsigned long long));
-      const unsigned long long none = ~0ull;
-      SBAMD_HIP(hipMemcpy(d_bad.ptr, &none, sizeof(none), hipMemcpyHostToDevice));
-      P.m = d_m.as<int64_t>();
-      P.bad = d_bad.as<unsigned long long>();
-      launch_pack_count(P, in.dtype == SBAMD_F64, st);
-      unsigned long long bad = none;
-      std::vector<int64_t> m(static_cast<size_t>(in.n));
-      SBAMD_HIP(hipMemcpyAsync(&bad, d_bad.ptr, sizeof(bad), hipMemcpyDeviceToHost, st));
-      if (in.n > 0)
-        SBAMD_HIP(hipMemcpyAsync(m.data(), d_m.ptr, m.size() * sizeof(int64_t), hipMemcpyDeviceToHost, st));
-      SBAMD_HIP(hipStreamSynchronize(st));
-      if (bad != none)
-      {
-        // the host rules on the first offending PCF give the message
-        const int64_t i = static_cast<int64_t>(bad);
-        const int64_t b = h_off[i], cnt = h_off[i + 1] - b;
-        std::vector<unsigned char> raw_t(static_cast<size_t>(cnt * es)), raw_v(raw_t.size());
-        for (int64_t k = 0; k < cnt; ++k)
-        {
-          const hipMemcpyKind kind = hipMemcpyDeviceToHost;
-          SBAMD_HIP(hipMemcpy(raw_t.data() + k * es, tp + (b + k) * in.t_stride, es, kind));
-          SBAMD_HIP(hipMemcpy(raw_v.data() + k * es, vp + (b + k) * in.v_stride, es, kind));
-        }
-        const int64_t one_off[2] = {0, cnt};
-        sbamd_pcf_set one;
-        one.n = 1;
-        one.offsets = one_off;
-        one.t = raw_t.data();
-        one.v = raw_v.data();
-        one.dtype = in.dtype;
-        validate_set(one);
-        fail(SBAMD_ERR_RUNTIME, "device validation flagged PCF " + std::to_string(i) +
-                                    " but the host rules accept it");
-      }
+      pack_on_gpu(ctx, s, h_off, tp, vp, in.t_stride, in.v_stride, in.dtype, nullptr);
+    }
 
-      std::vector<int64_t> perm;
-      sort_by_records(m, perm);
-      std::vector<int64_t> roff(static_cast<size_t>(in.n) + 1, 0), qinv(static_cast<size_t>(in.n));
-      for (int64_t q = 0; q < in.n; ++q)
+    // A device set straight from a file image of a pcf32 / pcf64 tensor or
+    // object: the payload (point counts and points as they lie) is copied to
+    // the device once and PCF i's points are read at its own byte offset.
+    void set_from_image(Ctx* ctx, const void* buf, uint64_t len, int32_t precision, Set& s)
+    {
+      sbamd_io_info info;
+      io_parse(buf, len, info);
+      if (info.base_format != 100)
+        fail(SBAMD_ERR_INVALID, "the file does not hold PCFs with floating-point times and values "
+                                "(element format (" + std::to_string(info.base_format) + ", " +
+                                    std::to_string(info.sub_format) + "))");
+      const uint64_t E = info.elements;
+      if (E > static_cast<uint64_t>(std::numeric_limits<int32_t>::max() / 2))
+        fail(SBAMD_ERR_INVALID, "PCF set too large (more than 2^30 PCFs)");
+      std::vector<uint64_t> counts(E), starts(E);
+      const uint64_t end = io_index(buf, len, info, counts.data(), starts.data());
+      const int32_t dtype = info.sub_format == 64 ? SBAMD_F64 : SBAMD_F32;
+      const int64_t es = info.sub_format / 8, pb = 2 * es;
+      std::vector<int64_t> h_off(E + 1, 0), pbase(E);
+      for (uint64_t e = 0; e < E; ++e)
+        h_off[e + 1] = h_off[e] + static_cast<int64_t>(counts[e]);
+      for (uint64_t e = 0; e < E; ++e)
+        pbase[e] = static_cast<int64_t>(starts[e] - info.payload) - h_off[e] * pb;
+      use_device(ctx);
+      s.ctx = ctx;
+      s.precision = precision;
+      s.in_dtype = dtype;
+      set_layout_for(precision, dtype, s.tt_f64, s.tv_f64);
+      s.n = static_cast<int64_t>(E);
+      s.points = h_off[E];
+      s.has_empty = false;
+      for (uint64_t e = 0; e < E; ++e)
+        s.has_empty |= counts[e] == 0;
+      DeviceBuffer d_img(static_cast<size_t>(end - info.payload));
+      if (d_img.bytes)
+        SBAMD_HIP(hipMemcpy(d_img.ptr, static_cast<const unsigned char*>(buf) + info.payload, d_img.bytes,
+                            hipMemcpyHostToDevice));
+      const unsigned char* tp = d_img.as<unsigned char>();
+      pack_on_gpu(ctx, s, h_off, tp, tp + es, pb, pb, dtype, &pbase);
+    }
+
+    // Large host sets are packed on the GPU (the same bytes as the host
+    // packer, one upload of t and v instead of a single-threaded pass).
+    constexpr int64_t GPU_PACK_MIN_POINTS = int64_t(1) << 16;
+
+    void make_set_auto(Ctx* ctx, const sbamd_pcf_set& in, int32_t precision, Set& s)
+    {
+      const int64_t M = in.n > 0 && in.offsets ? in.offsets[in.n] : 0;
+      if (M < GPU_PACK_MIN_POINTS)
       {
-        roff[q + 1] = roff[q] + m[perm[q]];
-        qinv[perm[q]] = q;
+        make_set(ctx, in, precision, s);
+        return;
       }
-      s.records = roff.back();
-      s.h_roff = roff;
-      s.h_perm = perm;
-      upload(s.roff, roff);
-      upload(s.perm, perm);
-      DeviceBuffer d_qinv;
-      upload(d_qinv, qinv);
-      s.rec = DeviceBuffer(static_cast<size_t>(s.records) * record_bytes(s.tt_f64));
-      s.src = DeviceBuffer(static_cast<size_t>(s.records) * sizeof(int64_t));
-      P.roff = s.roff.as<int64_t>();
-      P.qinv = d_qinv.as<int64_t>();
-      P.rec = s.rec.ptr;
-      P.src = s.src.as<int64_t>();
-      launch_pack_records(P, in.dtype == SBAMD_F64, st);
-      SBAMD_HIP(hipStreamSynchronize(st));
+      sbamd_pcf_view view;
+      view.n = in.n;
+      view.offsets = in.offsets;
+      view.t = in.t;
+      view.v = in.v;
+      view.t_stride = view.v_stride = in.dtype == SBAMD_F64 ? 8 : 4;
+      view.dtype = in.dtype;
+      view.location = SBAMD_HOST;
+      make_set_gpu(ctx, view, precision, s);
     }
 
     std::vector<int64_t> sorted_counts(const Set& s)
@@ -687,6 +769,59 @@ extern "C"
       auto s = std::make_unique<sbamd_set>();
       make_set_gpu(ctx, *view, precision, *s);
       *out = s.release();
+    });
+  }
+
+  int sbamd_set_create_image(sbamd_ctx* ctx, const void* buf, uint64_t len, int32_t precision, sbamd_set** out)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      need(out, "out");
+      if (!buf && len)
+        fail(SBAMD_ERR_INVALID, "buf is NULL");
+      *out = nullptr;
+      auto s = std::make_unique<sbamd_set>();
+      set_from_image(ctx, buf, len, precision, *s);
+      *out = s.release();
+    });
+  }
+
+  int sbamd_io_parse(const void* buf, uint64_t len, sbamd_io_info* info)
+  {
+    return guarded([&] {
+      need(info, "info");
+      io_parse(buf, len, *info);
+    });
+  }
+
+  int sbamd_io_index(const void* buf, uint64_t len, const sbamd_io_info* info, uint64_t* counts,
+                     uint64_t* starts, uint64_t* end)
+  {
+    return guarded([&] {
+      need(info, "info");
+      const uint64_t e = io_index(buf, len, *info, counts, starts);
+      if (end)
+        *end = e;
+    });
+  }
+
+  int sbamd_io_header(uint32_t kind, int32_t base_format, int32_t sub_format, int32_t ndim,
+                      const uint64_t* shape, void* out, uint64_t cap, uint64_t* size)
+  {
+    return guarded([&] {
+      const uint64_t n = io_header(kind, base_format, sub_format, ndim, shape, out, cap);
+      if (size)
+        *size = n;
+    });
+  }
+
+  int sbamd_io_write_pcfs(int64_t n, const int64_t* offsets, const void* t, const void* v, int32_t dtype,
+                          void* out, uint64_t cap, uint64_t* size)
+  {
+    return guarded([&] {
+      const uint64_t b = io_write_pcfs(n, offsets, t, v, dtype, out, cap);
+      if (size)
+        *size = b;
     });
   }
 
@@ -1456,9 +1591,9 @@ extern "C"
           if (total > 0)
           {
             sbamd_set rs, cs;
-            make_set(ctx, rcopy, precision, rs);
+            make_set_auto(ctx, rcopy, precision, rs);
             if (has_cols)
-              make_set(ctx, ccopy, precision, cs);
+              make_set_auto(ctx, ccopy, precision, cs);
             sbamd_plan* pl = nullptr;
             int rc = sbamd_plan_create(ctx, &rs, has_cols ? &cs : nullptr, op, p, out_kind, nullptr, &pl);
             if (rc != SBAMD_OK)

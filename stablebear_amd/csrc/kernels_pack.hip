@@ -11,7 +11,9 @@
 // when the next point has a later time or it is the last one, and its record
 // index is the number of kept points before it (ballot + popcount). Inputs
 // are strided views, so the same kernels read SoA arrays (stride = element
-// size) and AoS {t, v} point arrays (stride = 2 x element size).
+// size) and AoS {t, v} point arrays (stride = 2 x element size); with
+// per-PCF byte bases they also read a file image as it lies, where each
+// PCF's points follow its own point count.
 #include "sbamd_internal.hpp"
 
 #include <cmath>
@@ -45,6 +47,7 @@ namespace sbamd
         P.m[i] = 1; // the zero PCF: one record
       return;
     }
+    const unsigned char* const tp = P.t + (P.pbase ? P.pbase[i] : 0);
     int64_t cnt = 0;
     bool bad = false;
     for (int64_t c = b; c < e; c += 64)
@@ -54,14 +57,14 @@ namespace sbamd
       bool keep = false;
       if (in)
       {
-        const double t = load_at<T>(P.t, k, P.t_stride);
+        const double t = load_at<T>(tp, k, P.t_stride);
         if (!isfinite(t))
           bad = true;
         if (k == b && t != 0.0)
           bad = true;
-        if (k > b && t < load_at<T>(P.t, k - 1, P.t_stride))
+        if (k > b && t < load_at<T>(tp, k - 1, P.t_stride))
           bad = true;
-        keep = k + 1 == e || t < load_at<T>(P.t, k + 1, P.t_stride);
+        keep = k + 1 == e || t < load_at<T>(tp, k + 1, P.t_stride);
       }
       cnt += __popcll(__ballot(keep));
     }
@@ -81,6 +84,9 @@ namespace sbamd
       return;
     const int64_t b = P.off[i], e = P.off[i + 1];
     const int64_t r0 = P.roff[P.qinv[i]];
+    const int64_t pb = P.pbase ? P.pbase[i] : 0;
+    const unsigned char* const tp = P.t + pb;
+    const unsigned char* const vp = P.v + pb;
     const int rb = P.tt_f64 ? 16 : 8;
     unsigned char* const rec = static_cast<unsigned char*>(P.rec);
     // record r = {end time, value} in the set's types; the value sits at
@@ -121,10 +127,10 @@ namespace sbamd
       bool keep = false;
       if (in)
       {
-        const double t = load_at<T>(P.t, k, P.t_stride);
+        const double t = load_at<T>(tp, k, P.t_stride);
         if (!last)
         {
-          const double tn = load_at<T>(P.t, k + 1, P.t_stride);
+          const double tn = load_at<T>(tp, k + 1, P.t_stride);
           keep = t < tn;
           tend = tn + 0.0; // + 0.0 maps -0.0 to +0.0
         }
@@ -137,7 +143,7 @@ namespace sbamd
         const int64_t r = r0 + base +
                           __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-        put(r, tend, load_at<T>(P.v, k, P.v_stride));
+        put(r, tend, load_at<T>(vp, k, P.v_stride));
         P.src[r] = k;
       }
       base += __popcll(mask);

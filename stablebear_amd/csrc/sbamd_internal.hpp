@@ -287,11 +287,21 @@ namespace sbamd
     unsigned long long* bad;   // pass 1: lowest PCF index breaking a rule
     const int64_t* roff;       // pass 2: sorted record offsets (n + 1)
     const int64_t* qinv;       // pass 2: PCF -> sorted position
+    const int64_t* pbase;      // optional: PCF i's points are at byte pbase[i] + k * stride
     void* rec;                 // pass 2: records
     int64_t* src;              // pass 2: record -> point (-1: zero PCF)
     int32_t tt_f64, tv_f64;    // record types
   };
   void launch_pack_count(const PackParams& P, int f64, hipStream_t stream);
+
+  // The reference's file format (io.cpp).
+  void io_parse(const void* buf, uint64_t len, sbamd_io_info& info);
+  uint64_t io_index(const void* buf, uint64_t len, const sbamd_io_info& info, uint64_t* counts,
+                    uint64_t* starts);
+  uint64_t io_header(uint32_t kind, int32_t base_format, int32_t sub_format, int32_t ndim,
+                     const uint64_t* shape, void* out, uint64_t cap);
+  uint64_t io_write_pcfs(int64_t n, const int64_t* offsets, const void* t, const void* v, int32_t dtype,
+                         void* out, uint64_t cap);
   void launch_pack_records(const PackParams& P, int f64, hipStream_t stream);
   void launch_rectangles(const double* ft, const double* fv, int64_t mf, const double* gt,
                          const double* gv, int64_t mg, double a, double b, double* out,

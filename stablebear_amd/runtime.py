@@ -298,6 +298,21 @@ class DeviceSet:
         self.n, self.records, self.points = n.value, rec.value, pts.value
 
     @classmethod
+    def from_handle(cls, handle, precision: str, device: int | None = None, keep=None):
+        """Wrap a set handle created by the C-ABI (the object owns it)."""
+        self = cls.__new__(cls)
+        self.ctx = context(device)
+        self.csr = None
+        self.precision = precision
+        self._keep = keep
+        self.handle = handle
+        n, rec, pts = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        L.check(L.lib().sbamd_set_info(handle, ctypes.byref(n), ctypes.byref(rec),
+                                       ctypes.byref(pts)))
+        self.n, self.records, self.points = n.value, rec.value, pts.value
+        return self
+
+    @classmethod
     def from_device(cls, offsets, t, v, precision: str = "fp64", device: int | None = None):
         """Set from CSR arrays already in device memory (torch tensors on the
         GPU: offsets int64 (n+1), t and v float32/float64 of one dtype, any

@@ -1,0 +1,182 @@
+"""The reference's binary file format (include/sbear/io.hpp, io/tensor_io.hpp,
+io/pcf_io.hpp, io/point_io.hpp, io/compressed_matrix_io.hpp) through the
+native reader / writer (``sbamd_io_*``): files assembled byte by byte from the
+reference's layout are read back exactly (including format version 1, which
+has no platform string), the writer's bytes follow that layout (the facts the
+reference's own tests assert: test/test_io_readwrite.cpp:126-330,
+test/test_tensor_io_core.cpp:19-80), every supported type round-trips, and
+corrupted, truncated and foreign files raise the reference's errors
+(test/python/test_serialization_robustness.py)."""
+
+import io as pyio
+import struct
+
+import numpy as np
+import pytest
+
+import stablebear_amd as sb
+from stablebear_amd import workloads
+
+
+def _str(s: bytes) -> bytes:
+    return struct.pack("<Q", len(s)) + s
+
+
+def _header(version=2, kind=1, base=100, sub=64):
+    h = b"\x01MPCF" + b"e" + struct.pack("<i", version) + _str(b"9.9.9") + _str(b"today")
+    if version >= 2:
+        h += _str(b"linux-x86_64")
+    return h + struct.pack("<Iii", kind, base, sub)
+
+
+def _shape(shape):
+    strides = []
+    s = 1
+    for d in reversed(shape):
+        strides.append(s)
+        s *= d
+    strides = strides[::-1]
+    return struct.pack("<Q", len(shape)) + b"".join(struct.pack("<QQ", d, st)
+                                                    for d, st in zip(shape, strides))
+
+
+def _pcf_bytes(pts, fmt="<dd"):
+    return struct.pack("<Q", len(pts)) + b"".join(struct.pack(fmt, t, v) for t, v in pts)
+
+
+PCFS = [[(0.0, 1.0), (1.0, 2.0), (3.0, 0.0)], [(0.0, 5.0), (2.5, 0.0)], [(0.0, 0.0)],
+        [(0.0, -1.0), (0.5, 4.0), (0.5, 2.0), (7.0, 0.0)]]
+
+
+@pytest.mark.parametrize("version", [1, 2])
+def test_reads_a_hand_assembled_pcf64_tensor(version):
+    img = _header(version) + _shape((2, 2)) + b"".join(_pcf_bytes(p) for p in PCFS)
+    X = sb.io.loads(img)
+    assert isinstance(X, sb.PcfTensor) and X.shape == (2, 2) and X.dtype is sb.pcf64
+    for k, p in enumerate(PCFS):  # row-major element order
+        np.testing.assert_array_equal(X[k // 2, k % 2].to_numpy(), np.array(p))
+
+
+def test_reads_hand_assembled_pcf32_object_and_matrices():
+    img = _header(kind=2, base=100, sub=32) + _pcf_bytes(PCFS[0], "<ff")
+    f = sb.io.loads(img)
+    assert isinstance(f, sb.Pcf) and f.dtype is sb.pcf32
+    np.testing.assert_array_equal(f.to_numpy(), np.array(PCFS[0], dtype=np.float32))
+    # distance matrix object: u64 n, then n(n-1)/2 values (index r(r-1)/2 + c)
+    img = _header(kind=2, base=1120, sub=64) + struct.pack("<Q3d", 3, 2.0, 4.0, 6.0)
+    D = sb.io.loads(img)
+    assert isinstance(D, sb.DistanceMatrix) and D.size == 3
+    assert (D[1, 0], D[2, 0], D[2, 1]) == (2.0, 4.0, 6.0)
+    # symmetric matrix object: u64 n, then n(n+1)/2 values (index r(r+1)/2 + c)
+    img = _header(kind=2, base=1100, sub=32) + struct.pack("<Q3f", 2, 1.0, 2.0, 3.0)
+    K = sb.io.loads(img)
+    assert isinstance(K, sb.SymmetricMatrix) and K.size == 2
+    assert (K[0, 0], K[1, 0], K[1, 1]) == (1.0, 2.0, 3.0)
+
+
+def test_reads_hand_assembled_numeric_tensors():
+    img = _header(base=1, sub=64) + _shape((2, 3)) + struct.pack("<6d", *range(6))
+    T = sb.io.loads(img)
+    assert isinstance(T, sb.FloatTensor)
+    np.testing.assert_array_equal(T.to_numpy(), np.arange(6.0).reshape(2, 3))
+    img = _header(base=2, sub=32) + _shape((4,)) + struct.pack("<4i", 1, -2, 3, -4)
+    np.testing.assert_array_equal(sb.io.loads(img), np.array([1, -2, 3, -4], dtype=np.int32))
+    img = _header(base=4, sub=8) + _shape((3,)) + bytes([1, 0, 1])
+    np.testing.assert_array_equal(sb.io.loads(img), np.array([True, False, True]))
+
+
+def test_writer_follows_the_reference_layout():
+    X = sb.PcfTensor([sb.Pcf(np.array(p)) for p in PCFS[:2]])
+    data = sb.io.dumps(X)
+    assert data[:5] == b"\x01MPCF" and data[5:6] == b"e"
+    assert struct.unpack_from("<i", data, 6)[0] == 2  # format version at byte 6
+    pos = 10
+    for _ in range(3):  # version, build date, platform
+        n = struct.unpack_from("<Q", data, pos)[0]
+        pos += 8 + n
+    assert struct.unpack_from("<Iii", data, pos) == (1, 100, 64)
+    pos += 12
+    assert data[pos:] == _shape((2,)) + _pcf_bytes(PCFS[0]) + _pcf_bytes(PCFS[1])
+
+
+def test_format_codes_of_the_reference_table():
+    cases = [(np.zeros(2, np.float32), (1, 32)), (np.zeros(2, np.float64), (1, 64)),
+             (np.zeros(2, np.int32), (2, 32)), (np.zeros(2, np.int64), (2, 64)),
+             (np.zeros(2, np.uint32), (3, 32)), (np.zeros(2, np.uint64), (3, 64)),
+             (np.zeros(2, np.bool_), (4, 8)),
+             (sb.zeros((2,), sb.pcf32), (100, 32)), (sb.zeros((2,), sb.pcf64), (100, 64)),
+             (sb.zeros((2,), sb.pcf32i), (101, 32)), (sb.zeros((2,), sb.pcf64i), (101, 64))]
+    for item, code in cases:
+        data = sb.io.dumps(item)
+        pos = 10
+        for _ in range(3):
+            pos += 8 + struct.unpack_from("<Q", data, pos)[0]
+        assert struct.unpack_from("<Iii", data, pos) == (1,) + code
+
+
+@pytest.mark.parametrize("dtype", [sb.pcf32, sb.pcf64])
+def test_pcf_tensor_round_trips(dtype):
+    off, t, v = workloads.c1(40)
+    np_dt = np.float32 if dtype is sb.pcf32 else np.float64
+    pcfs = [sb.Pcf(np.stack([t[off[i]:off[i + 1]], v[off[i]:off[i + 1]]], 1).astype(np_dt))
+            for i in range(40)]
+    X = sb.PcfTensor(pcfs)
+    for shape in [(40,), (5, 8), (2, 2, 10)]:
+        arr = np.empty(40, dtype=object)
+        arr[:] = pcfs
+        Xs = sb.PcfTensor(arr.reshape(shape))
+        buf = pyio.BytesIO()
+        sb.save(Xs, buf)
+        buf.seek(0)
+        Y = sb.load(buf)
+        assert Y.shape == shape and Y.dtype is dtype
+        assert all(a == b for a, b in zip(Xs._a.ravel(), Y._a.ravel()))
+    assert sb.io.loads(sb.io.dumps(sb.io.loads(sb.io.dumps(X))))._a.size == 40  # two round trips
+
+
+def test_int_pcfs_matrices_and_objects_round_trip(tmp_path):
+    Xi = sb.zeros((3,), dtype=sb.pcf32i)
+    Yi = sb.io.loads(sb.io.dumps(Xi))
+    assert isinstance(Yi, sb.IntPcfTensor) and Yi.dtype is sb.pcf32i
+    f = sb.Pcf(np.array([[0.0, 42.0], [1.0, 0.0]]))
+    path = tmp_path / "f.sb"
+    sb.save(f, str(path))
+    assert sb.load(str(path)) == f
+    D = sb.DistanceMatrix.from_dense(np.array([[0.0, 1.0, 2.0], [1.0, 0.0, 3.0], [2.0, 3.0, 0.0]]))
+    assert sb.io.loads(sb.io.dumps(D)) == D
+    arr = np.empty(2, dtype=object)
+    arr[0] = D
+    arr[1] = sb.DistanceMatrix(0)
+    back = sb.io.loads(sb.io.dumps(arr))
+    assert back.shape == (2,) and back[0] == D and back[1].size == 0
+
+
+def test_corrupted_truncated_and_foreign_files_raise():
+    X = sb.zeros((3,), dtype=sb.pcf64)
+    good = sb.io.dumps(X)
+    with pytest.raises(RuntimeError, match="Unexpected EOF"):
+        sb.io.loads(b"")
+    with pytest.raises(RuntimeError, match="Unrecognized file format"):
+        sb.io.loads(b"\x00\x01\x02\x03\xff\xfe\xfd" * 10)
+    with pytest.raises(RuntimeError):
+        sb.io.loads(good[: len(good) // 2])
+    bad = bytearray(good)
+    bad[6:10] = struct.pack("<i", 9999)
+    with pytest.raises(RuntimeError, match="format version 9999"):
+        sb.io.loads(bytes(bad))
+    big = bytearray(good)
+    big[10:18] = struct.pack("<Q", 40000)  # version string longer than the reader's limit
+    with pytest.raises(RuntimeError, match="exceeds the 32767 limit"):
+        sb.io.loads(bytes(big))
+    img = _header(base=1, sub=64) + struct.pack("<QQQ", 1, 2, 7) + struct.pack("<2d", 1, 2)
+    with pytest.raises(RuntimeError, match="Incorrect strides"):
+        sb.io.loads(img)
+    img = _header(base=1000, sub=32) + _shape((1,))
+    with pytest.raises(TypeError, match="point clouds"):
+        sb.io.loads(img)
+
+
+def test_rank0_tensor_has_no_elements():
+    img = _header(base=1, sub=64) + struct.pack("<Q", 0)
+    T = sb.io.loads(img)
+    assert T.size == 0
