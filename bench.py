@@ -544,9 +544,20 @@ def _ingest_leg(args):
     d_v = torch.from_numpy(v).cuda()
     gd_s, gd_set = wall(lambda: runtime.DeviceSet.from_device(d_off, d_t, d_v, "fp32c"), 3)
     _progress(f"ingest: GPU packer from device arrays {gd_s:.3f} s")
+    # from a file image in the reference's format (in memory): parse to CSR on
+    # the host and pack on the host, against the image straight to the GPU
+    import stablebear_amd as sb
+
+    img = sb.io.dumps_csr(off, t, v)
+    fh_s, fh_set = wall(lambda: runtime.DeviceSet(runtime.csr_from_arrays(*sb.io.load_csr(img)),
+                                                  "fp32c"), 1)
+    _progress(f"ingest: file image, host parse + host packer {fh_s:.3f} s")
+    fg_s, fg_set = wall(lambda: sb.io.load_device_set(img, "fp32c"), 3)
+    _progress(f"ingest: file image straight to the GPU packer {fg_s:.3f} s")
     a = host_set.export()
-    identical = all(np.array_equal(x, y) for x, y in zip(a, gh_set.export()))
-    identical = identical and all(np.array_equal(x, y) for x, y in zip(a, gd_set.export()))
+    identical = True
+    for other in (gh_set, gd_set, fh_set, fg_set):
+        identical = identical and all(np.array_equal(x, y) for x, y in zip(a, other.export()))
     in_bytes = M * 16
     return {"ingest": {
         "workload": f"one side of L1 cdist: N={args.ingest_n}, m~U{{500..5000}}, FP64 t and v, "
@@ -555,6 +566,8 @@ def _ingest_leg(args):
         "host_pack_s": host_s, "gpu_pack_from_host_s": gh_s, "gpu_pack_from_device_s": gd_s,
         "host_over_gpu_from_host": host_s / gh_s, "host_over_gpu_from_device": host_s / gd_s,
         "gpu_from_device_input_GBs": in_bytes / gd_s / 1e9,
+        "file_image_bytes": int(img.size), "file_host_parse_and_pack_s": fh_s,
+        "file_to_gpu_pack_s": fg_s, "file_host_over_gpu": fh_s / fg_s,
         "layouts_identical": bool(identical)}}
 
 

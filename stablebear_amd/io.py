@@ -124,6 +124,50 @@ def dumps(item) -> bytes:
     raise TypeError(f"Unsupported item type {type(item).__name__}")
 
 
+def dumps_csr(offsets, t, v) -> np.ndarray:
+    """The file image (uint8 array) of a 1-D pcf32 / pcf64 tensor given in CSR
+    form (offsets n+1, times, values), written natively without Pcf objects."""
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    t = np.ascontiguousarray(t)
+    np_dt = np.float64 if t.dtype == np.float64 else np.float32
+    t = t.astype(np_dt, copy=False)
+    v = np.ascontiguousarray(v, dtype=np_dt)
+    n = len(offsets) - 1
+    head = np.frombuffer(_header(L.IO_TENSOR, 100, 64 if np_dt == np.float64 else 32, (n,)),
+                         dtype=np.uint8)
+    dt = L.F64 if np_dt == np.float64 else L.F32
+    size = ctypes.c_uint64()
+    args = (n, offsets.ctypes.data_as(L.c_i64p), t.ctypes.data if t.size else None,
+            v.ctypes.data if v.size else None, dt)
+    L.check(L.lib().sbamd_io_write_pcfs(*args, None, 0, ctypes.byref(size)))
+    out = np.empty(head.size + size.value, dtype=np.uint8)
+    out[: head.size] = head
+    body = out[head.size:]
+    L.check(L.lib().sbamd_io_write_pcfs(*args, body.ctypes.data if body.size else None,
+                                        size.value, ctypes.byref(size)))
+    return out
+
+
+def load_csr(file):
+    """(offsets, t, v) numpy arrays of a file holding a tensor or object of
+    pcf32 / pcf64 (elements in row-major order), without Pcf objects."""
+    img = _image(file)
+    info = _info(img)
+    if info.base_format != 100:
+        raise TypeError("the file does not hold PCFs with floating-point times and values")
+    counts, starts = _index(img, info)
+    np_dt = np.float64 if info.sub_format == 64 else np.float32
+    offsets = np.zeros(len(counts) + 1, dtype=np.int64)
+    np.cumsum(counts.astype(np.int64), out=offsets[1:])
+    pb = 2 * np.dtype(np_dt).itemsize
+    if offsets[-1]:
+        pts = np.concatenate([img[int(s): int(s) + int(c) * pb] for c, s in zip(counts, starts)])
+        pts = pts.view(np_dt).reshape(-1, 2)
+    else:
+        pts = np.zeros((0, 2), dtype=np_dt)
+    return offsets, np.ascontiguousarray(pts[:, 0]), np.ascontiguousarray(pts[:, 1])
+
+
 def save(item, file) -> None:
     """Save a tensor or object (``Pcf``, ``DistanceMatrix``, ``SymmetricMatrix``)
     in the reference's binary format. ``file`` is a path or a binary file."""
@@ -136,7 +180,12 @@ def save(item, file) -> None:
 
 
 def _image(file) -> np.ndarray:
-    """The file's bytes as a uint8 array (memory-mapped for paths)."""
+    """The file's bytes as a uint8 array (memory-mapped for paths; images
+    already in memory are used as they are)."""
+    if isinstance(file, np.ndarray):
+        return file.view(np.uint8).ravel()
+    if isinstance(file, (bytes, bytearray, memoryview)):
+        return np.frombuffer(file, dtype=np.uint8)
     if isinstance(file, (str, os.PathLike)):
         if os.path.getsize(file) == 0:
             return np.zeros(0, dtype=np.uint8)
@@ -215,10 +264,11 @@ def load(file):
 
 
 def load_device_set(file, precision: str = "fp64", device: int | None = None):
-    """A device set (``runtime.DeviceSet``) straight from a file holding a
-    tensor (any shape, row-major order) or an object of pcf32 / pcf64: the
-    payload is copied to the GPU once as it lies and validated and packed
-    there, with no per-PCF host objects."""
+    """A device set (``runtime.DeviceSet``) straight from a file (path, file
+    object, or an image in memory) holding a tensor (any shape, row-major
+    order) or an object of pcf32 / pcf64: the payload is copied to the GPU
+    once as it lies and validated and packed there, with no per-PCF host
+    objects."""
     from . import runtime
 
     img = _image(file)

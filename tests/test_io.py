@@ -180,3 +180,17 @@ def test_rank0_tensor_has_no_elements():
     img = _header(base=1, sub=64) + struct.pack("<Q", 0)
     T = sb.io.loads(img)
     assert T.size == 0
+
+
+@pytest.mark.parametrize("np_dt", [np.float32, np.float64])
+def test_csr_level_writer_and_reader(np_dt):
+    off, t, v = workloads.c1(60)
+    t, v = t.astype(np_dt), v.astype(np_dt)
+    img = sb.io.dumps_csr(off, t, v)
+    pcfs = [sb.Pcf(np.stack([t[off[i]:off[i + 1]], v[off[i]:off[i + 1]]], 1)) for i in range(60)]
+    # the same file as the object-level writer's for the same PCFs
+    assert img.tobytes() == sb.io.dumps(sb.PcfTensor(pcfs))
+    o2, t2, v2 = sb.io.load_csr(img)
+    np.testing.assert_array_equal(o2, off)
+    np.testing.assert_array_equal(t2, t)
+    np.testing.assert_array_equal(v2, v)
