@@ -282,6 +282,24 @@ int sbamd_iterate_rectangles(sbamd_ctx* ctx, const sbamd_pcf_set* f, const sbamd
                              double a, double b, void* out, int64_t cap, int64_t* count);
 
 /* ---- host-only planning helpers (no device needed) ---- */
+/* ---- row reductions of a pairwise output in device memory ----
+ * `out_kind` SBAMD_OUT_DISTMAT: the compressed lower triangle of an n x n
+ * distance matrix (n = n_rows; row i runs over every j != i); SBAMD_OUT_DENSE:
+ * a row-major n_rows x n_cols matrix. Elements of `dtype` (SBAMD_F32 /
+ * SBAMD_F64); asynchronous on `stream` (NULL: the null stream). No reference
+ * counterpart: callers of the reference reduce the returned matrices on the
+ * host (D.mean(axis=1) in its cdist benchmark); these generalise the fused
+ * row sums of sbamd_plan_execute to any output already on the device.
+ * Per row: sum, minimum and maximum in FP64 (any output may be NULL; the
+ * mean is sum / entries). */
+int sbamd_rows_reduce(sbamd_ctx* ctx, int32_t out_kind, int64_t n_rows, int64_t n_cols, const void* d_device,
+                      int32_t dtype, double* sum_device, double* min_device, double* max_device, void* stream);
+/* The k nearest columns of every row (1 <= k <= 32): values (`dtype`) and
+ * column indices, n_rows x k row-major, ascending by (value, column); rows
+ * with fewer than k entries are padded with +inf and -1. */
+int sbamd_rows_knn(sbamd_ctx* ctx, int32_t out_kind, int64_t n_rows, int64_t n_cols, const void* d_device,
+                   int32_t dtype, int32_t k, void* values_device, int64_t* index_device, void* stream);
+
 /* ---- the reference's binary file format (include/sbear/io.hpp:27-158,
  * io/tensor_io.hpp, io/pcf_io.hpp, io/compressed_matrix_io.hpp) ----
  * Files hold one tensor or one object behind a header ("\1MPCF", 'e', format

@@ -14,7 +14,7 @@ one launch, hipGraph-capturable), autograd (``stablebear_amd.autograd``) and
 an autotuner for pair-tile shapes.
 """
 
-from . import io, system
+from . import io, rows, system
 from .io import load, save
 from .batched import l2_kernel_batched, pdist_batched
 from .distance import Distance_f32_f32, Distance_f64_f64, cdist, lp_distance, pdist
@@ -36,5 +36,5 @@ __all__ = [
     "Distance_f32_f32", "Distance_f64_f64", "InnerProduct_f32_f32", "InnerProduct_f64_f64",
     "Norms_f32_f32", "Norms_f64_f64",
     "dtype", "float32", "float64", "int32", "int64", "pcf32", "pcf64", "pcf32i", "pcf64i",
-    "system", "io", "save", "load",
+    "system", "io", "rows", "save", "load",
 ]

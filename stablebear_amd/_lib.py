@@ -95,6 +95,10 @@ _SIGS = {
     "sbamd_set_create_gpu": (None, [vp, ctypes.POINTER(PcfView), ctypes.c_int32,
                                     ctypes.POINTER(vp)]),
     "sbamd_set_export": (None, [vp, vp, c_i64p, c_i64p, c_i64p, c_i32p]),
+    "sbamd_rows_reduce": (None, [vp, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, vp, ctypes.c_int32,
+                                 vp, vp, vp, vp]),
+    "sbamd_rows_knn": (None, [vp, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, vp, ctypes.c_int32,
+                              ctypes.c_int32, vp, vp, vp]),
     "sbamd_set_create_image": (None, [vp, vp, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(vp)]),
     "sbamd_io_parse": (None, [vp, ctypes.c_uint64, ctypes.POINTER(IoInfo)]),
     "sbamd_io_index": (None, [vp, ctypes.c_uint64, ctypes.POINTER(IoInfo), c_u64p, c_u64p, c_u64p]),

@@ -19,7 +19,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libstablebear_amd.so")
 
 SOURCES = ["api.cpp", "pack.cpp", "io.cpp", "kernels_fwd.hip", "kernels_bwd.hip", "kernels_misc.hip",
-           "kernels_plain.hip", "kernels_long.hip", "kernels_norm.hip", "kernels_pack.hip"]
+           "kernels_plain.hip", "kernels_long.hip", "kernels_norm.hip", "kernels_pack.hip",
+           "kernels_rows.hip"]
 
 ARCH = os.environ.get("SBAMD_OFFLOAD_ARCH", "gfx950")
 

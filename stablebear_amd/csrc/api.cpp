@@ -425,6 +425,31 @@ namespace sbamd
       pack_on_gpu(ctx, s, h_off, tp, tp + es, pb, pb, dtype, &pbase);
     }
 
+    RowsParams rows_params(int32_t out_kind, int64_t n_rows, int64_t n_cols, const void* d, int32_t dtype)
+    {
+      if (out_kind != SBAMD_OUT_DISTMAT && out_kind != SBAMD_OUT_DENSE)
+        fail(SBAMD_ERR_INVALID, "row reductions take SBAMD_OUT_DISTMAT or SBAMD_OUT_DENSE matrices");
+      if (dtype != SBAMD_F32 && dtype != SBAMD_F64)
+        fail(SBAMD_ERR_INVALID, "dtype must be SBAMD_F32 or SBAMD_F64");
+      if (out_kind == SBAMD_OUT_DISTMAT)
+        n_cols = n_rows;
+      if (n_rows < 0 || n_cols < 0)
+        fail(SBAMD_ERR_INVALID, "matrix extents must be non-negative");
+      if (n_rows > std::numeric_limits<int32_t>::max() || n_cols > std::numeric_limits<int32_t>::max() - 1)
+        fail(SBAMD_ERR_INVALID, "matrix extents must be below 2^31");
+      const bool any = out_kind == SBAMD_OUT_DISTMAT ? n_rows > 1 : n_rows > 0 && n_cols > 0;
+      if (any)
+        need(d, "d_device");
+      RowsParams P;
+      std::memset(&P, 0, sizeof(P));
+      P.d = d;
+      P.n_rows = n_rows;
+      P.n_cols = n_cols;
+      P.compressed = out_kind == SBAMD_OUT_DISTMAT;
+      P.f64 = dtype == SBAMD_F64;
+      return P;
+    }
+
     // Large host sets are packed on the GPU (the same bytes as the host
     // packer, one upload of t and v instead of a single-threaded pass).
     constexpr int64_t GPU_PACK_MIN_POINTS = int64_t(1) << 16;
@@ -783,6 +808,41 @@ extern "C"
       auto s = std::make_unique<sbamd_set>();
       set_from_image(ctx, buf, len, precision, *s);
       *out = s.release();
+    });
+  }
+
+  int sbamd_rows_reduce(sbamd_ctx* ctx, int32_t out_kind, int64_t n_rows, int64_t n_cols, const void* d_device,
+                        int32_t dtype, double* sum_device, double* min_device, double* max_device, void* stream)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      RowsParams P = rows_params(out_kind, n_rows, n_cols, d_device, dtype);
+      P.sum = sum_device;
+      P.min = min_device;
+      P.max = max_device;
+      use_device(ctx);
+      launch_rows_reduce(P, pick_stream(ctx, stream));
+    });
+  }
+
+  int sbamd_rows_knn(sbamd_ctx* ctx, int32_t out_kind, int64_t n_rows, int64_t n_cols, const void* d_device,
+                     int32_t dtype, int32_t k, void* values_device, int64_t* index_device, void* stream)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      if (k < 1 || k > 32)
+        fail(SBAMD_ERR_INVALID, "k must be in [1, 32]");
+      RowsParams P = rows_params(out_kind, n_rows, n_cols, d_device, dtype);
+      if (P.n_rows > 0)
+      {
+        need(values_device, "values_device");
+        need(index_device, "index_device");
+      }
+      P.k = k;
+      P.knn_v = values_device;
+      P.knn_i = index_device;
+      use_device(ctx);
+      launch_rows_knn(P, pick_stream(ctx, stream));
     });
   }
 

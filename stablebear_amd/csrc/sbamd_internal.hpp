@@ -294,6 +294,23 @@ namespace sbamd
   };
   void launch_pack_count(const PackParams& P, int f64, hipStream_t stream);
 
+  // Row reductions of a pairwise output in device memory (kernels_rows.hip).
+  struct RowsParams
+  {
+    const void* d;        // dense n_rows x n_cols, or compressed lower triangle (n_rows = n)
+    int64_t n_rows, n_cols;
+    int32_t compressed;   // 1: distance-matrix storage, rows over j != i
+    int32_t f64;          // element type
+    double* sum;          // optional per-row outputs (FP64)
+    double* min;
+    double* max;
+    int32_t k;            // k nearest: values (element type) and columns
+    void* knn_v;
+    int64_t* knn_i;
+  };
+  void launch_rows_reduce(const RowsParams& P, hipStream_t stream);
+  void launch_rows_knn(const RowsParams& P, hipStream_t stream);
+
   // The reference's file format (io.cpp).
   void io_parse(const void* buf, uint64_t len, sbamd_io_info& info);
   uint64_t io_index(const void* buf, uint64_t len, const sbamd_io_info& info, uint64_t* counts,
