@@ -586,8 +586,8 @@ def _cdist_leg(args, stream):
     _progress(f"cdist: generated 2 x {n} PCFs ({int(X[0][-1]) + int(Y[0][-1])} points) "
               f"in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    sx = runtime.DeviceSet(runtime.csr_from_arrays(*X), "fp32c")
-    sy = runtime.DeviceSet(runtime.csr_from_arrays(*Y), "fp32c")
+    sx = runtime.DeviceSet(runtime.csr_from_arrays(*X), "fp32c", pack="gpu")
+    sy = runtime.DeviceSet(runtime.csr_from_arrays(*Y), "fp32c", pack="gpu")
     plan = runtime.Plan(sx, sy, op="l1")
     setup_s = time.perf_counter() - t0
     out = torch.empty(plan.out_count, dtype=torch.float32, device="cuda")
@@ -611,6 +611,7 @@ def _cdist_leg(args, stream):
     got = out[: r1 * n].double().cpu().numpy().reshape(r1, n)
     rel, _ = _errors(got.ravel(), exp.ravel())
     mean_rel = float(np.max(np.abs(row_means[:r1] - exp.mean(axis=1)) / np.abs(exp.mean(axis=1))))
+    rows_res = _rows_on_cdist(out, n, r1, stream)
     return {
         "leg": "cdist",
         "workload": f"L1 cdist {n} x {n}, m ~ U{{500..5000}}, Exp increments of mean 10, "
@@ -622,11 +623,52 @@ def _cdist_leg(args, stream):
         "setup_s": setup_s,
         "max_rel_err_vs_fp64_reference": rel,
         "row_mean_max_rel_err": mean_rel,
+        "rows": rows_res,
         "cpu_baseline": {"value": exp.size / cpu_s, "unit": "pairs/s", "cores": cores,
                          "kind": "reference" if use_ref else "port",
                          "sample": f"rows 0..{r1 - 1} against all {n} columns ({exp.size} pairs, "
                                    f"{cpu_s:.1f} s)"},
     }
+
+
+def _rows_on_cdist(out, n, r_check, stream, k=16):
+    """Row reductions of the dense n x n FP32 cdist output already in HBM:
+    per-row sum/min/max and the k nearest columns, timed with HIP events on
+    the launch stream; the first r_check rows' k nearest are compared with
+    numpy on the same outputs (exact: values and columns)."""
+    import torch
+
+    from stablebear_amd import _lib as L
+    from stablebear_amd import runtime
+
+    ctx = runtime.context()
+    st = torch.empty((3, n), dtype=torch.float64, device="cuda")
+    vals = torch.empty((n, k), dtype=torch.float32, device="cuda")
+    idx = torch.empty((n, k), dtype=torch.int64, device="cuda")
+    s = stream.cuda_stream
+
+    def stats():
+        L.check(L.lib().sbamd_rows_reduce(ctx, L.OUT_DENSE, n, n, out.data_ptr(), L.F32,
+                                          st[0].data_ptr(), st[1].data_ptr(), st[2].data_ptr(), s))
+
+    def nearest():
+        L.check(L.lib().sbamd_rows_knn(ctx, L.OUT_DENSE, n, n, out.data_ptr(), L.F32, k,
+                                       vals.data_ptr(), idx.data_ptr(), s))
+
+    stats_ms, _ = _events_ms(stats, 3, 1, stream)
+    knn_ms, _ = _events_ms(nearest, 3, 1, stream)
+    rows = out[: r_check * n].view(r_check, n).cpu().numpy()
+    ok = True
+    for i in range(r_check):
+        order = np.lexsort((np.arange(n), rows[i]))[:k]
+        ok = ok and np.array_equal(idx[i].cpu().numpy(), order)
+        ok = ok and np.array_equal(vals[i].cpu().numpy(), rows[i][order])
+    nbytes = n * n * 4
+    return {"k": k, "matrix_bytes": nbytes, "stats_ms": stats_ms, "knn_ms": knn_ms,
+            "stats_GBs": nbytes / stats_ms / 1e6, "knn_GBs": nbytes / knn_ms / 1e6,
+            "stats_hbm_frac": nbytes / stats_ms / 1e6 / PEAK_HBM_GBS,
+            "knn_hbm_frac": nbytes / knn_ms / 1e6 / PEAK_HBM_GBS,
+            f"knn_exact_on_{r_check}_rows": bool(ok)}
 
 
 def _long_leg(args, stream):

@@ -102,8 +102,10 @@ namespace sbamd
     }
   }
 
-  // Threads of a k-nearest workgroup: lists of K (value, column) pairs per
-  // thread in LDS stay within 24 KB.
+  // Threads of a k-nearest workgroup: the lists of K (value, column) pairs
+  // of all threads stay within 24 KB of LDS with FP64 values. (256 threads
+  // for K = 16 measured slower on a 50,000 x 50,000 matrix: more lists, more
+  // insertions, a longer merge.)
   template <int K>
   constexpr int knn_threads()
   {
