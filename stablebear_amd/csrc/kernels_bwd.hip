@@ -338,12 +338,12 @@ namespace sbamd
         H = fma(static_cast<double>(s_w[jj * 64 + lane]), s_g[jj], H);
     }
     // the row record open just before tlo
-    int64_t open = 0;
+    int32_t open = 0; // positions are below 2^31 (checked when the plan is made)
     if (act)
     {
       const int64_t a = B.ooff[i];
       const int64_t k = wv > 0 ? starts_below(B.key0, a, B.ooff[i + 1], tlo) : 0;
-      open = a + (k > 0 ? k - 1 : 0);
+      open = static_cast<int32_t>(a + (k > 0 ? k - 1 : 0));
     }
     // past the chunk's last event H_i is sum_j w_ij (last value of g_j),
     // taken exactly there rather than as the running sum of the jumps
@@ -393,9 +393,10 @@ namespace sbamd
     double nb = 0.0, blast = 0.0;
     int32_t npos = 0;
     auto next_own = [&]() {
-      const int32_t k = min(qptr, 63);
-      const double t = s_qt[k];
-      npos = s_qp[k];
+      // qptr = 64 (a lane that had all 64 events) reads s_eq[0] / s_ep[0],
+      // still this wave's LDS, and the value is discarded
+      const double t = s_qt[qptr];
+      npos = s_qp[qptr];
       nb = qptr < qend ? t : blast;
     };
     auto open_block = [&]() {
@@ -475,15 +476,15 @@ namespace sbamd
         const unsigned long long gom = __ballot(go);
         if (gom)
         {
-          double J = 0.0;
-          if (__ballot(go && __double_as_longlong(nb) == __double_as_longlong(tprev)))
-            J = jumps_at_tprev(ic, tprev);
           const int32_t ngo = __popcll(gom);
           if (nem + ngo > 64)
             flush();
           if (go)
           {
             // record npos - 1 ends at nb
+            double J = 0.0;
+            if (__double_as_longlong(nb) == __double_as_longlong(tprev))
+              J = jumps_at_tprev(ic, tprev);
             const double d = nb - tprev;
             const int32_t slot = nem + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(gom >> 32),
                                                                  __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(gom), 0u));
