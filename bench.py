@@ -235,6 +235,9 @@ def main(argv=None):
     ap.add_argument("--cdist-n", type=int, default=50000)
     ap.add_argument("--cdist-rows", type=int, default=48,
                     help="rows of the cdist leg checked against the reference walk on the CPU")
+    ap.add_argument("--norms", action="store_true",
+                    help="run only the lp_norm leg: per-PCF L1 and L2 norms of the N = 20,000 "
+                         "set (FP64 and FP32C) against the reference's norm loop on the host")
     ap.add_argument("--ingest", action="store_true",
                     help="run only the ingest leg: one side of the cdist workload (N = 50,000, "
                          "m ~ U{500..5000}, FP64) packed into a device set on the host and on "
@@ -277,6 +280,9 @@ def main(argv=None):
         return 0
     if args.ingest:
         print(json.dumps(_ingest_leg(args)), flush=True)
+        return 0
+    if args.norms:
+        print(json.dumps(_norms_leg(args, stream)), flush=True)
         return 0
 
     n = args.n
@@ -502,6 +508,47 @@ def _batched_leg(args, stream):
                          "sample": f"the {checked} smallest problems ({cpu_pairs} pairs, "
                                    f"{cpu_s:.1f} s)"},
     }
+
+
+def _norms_leg(args, stream):
+    """Per-PCF L_p norms (lp_norm) of the N-PCF L1 pdist set (m ~ U{50..500}):
+    device norms of a resident set timed with HIP events, checked against the
+    oracle (FP64 L1 bitwise, p = 2 to 1e-13, FP32C to 1e-6), next to the
+    reference's own norm loop (oracle/_ref, one host thread) on the same set.
+    Algorithmic bytes per launch: every record once (record bytes) plus, per
+    PCF, its record offset, permutation entry and output value."""
+    import torch
+
+    import oracle
+    from stablebear_amd import runtime, workloads
+
+    n = args.n
+    off, t, v = workloads.c2(n)
+    csr = runtime.csr_from_arrays(off, t, v)
+    res = {"leg": "lp_norm", "workload": f"per-PCF L1 and L2 norms, N={n}, m~U{{50..500}}"}
+    use_ref = oracle.ref_available()
+    for p in (1.0, 2.0):
+        t0 = time.perf_counter()
+        exp = oracle.lp_norms(p, off, t, v, use_ref=use_ref)
+        cpu_s = time.perf_counter() - t0
+        for prec in ("fp64", "fp32c"):
+            ds = runtime.DeviceSet(csr, prec, pack="gpu")
+            out = torch.empty(n, dtype=torch.float64, device="cuda")
+            ms, _ = _events_ms(lambda: ds.lp_norm(p, out.data_ptr(), True, stream.cuda_stream),
+                               20, 2, stream)
+            got = out.cpu().numpy()
+            rel = float(np.max(np.abs(got - exp) / np.abs(exp)))
+            rb = 16 if prec == "fp64" or t.dtype == np.float64 else 8
+            nbytes = ds.records * rb + n * 24
+            res[f"p{int(p)}_{prec}"] = {
+                "ms": ms, "norms_per_s": n / (ms * 1e-3), "bytes": nbytes,
+                "GBs": nbytes / ms / 1e6, "hbm_frac": nbytes / ms / 1e6 / PEAK_HBM_GBS,
+                "max_rel_err_vs_oracle": rel,
+                "bitexact": bool(np.array_equal(got, exp)) if prec == "fp64" else None}
+        res[f"p{int(p)}_cpu_baseline"] = {
+            "value": n / cpu_s, "unit": "norms/s", "cores": 1,
+            "kind": "reference" if use_ref else "port", "sample": f"all {n} PCFs ({cpu_s:.3f} s)"}
+    return res
 
 
 def _ingest_leg(args):

@@ -5,12 +5,25 @@
 // fma(0, |v|, acc) == acc); the last record's segment runs to the domain
 // end MAX in the time type, as the reference's does. FP64 sets accumulate
 // with one FMA per segment (the reference's operation sequence); FP32C sets
-// with the Kahan-compensated FP32 sum.
+// with the Kahan-compensated FP32 sum. p = 2 uses v * v and sqrt for the
+// reference's pow(|v|, 2) and pow(x, 1/2), both correctly rounded.
 #include "walk_common.hpp"
 
 namespace sbamd
 {
-  template <typename TT, typename TV, bool L1, bool KAHAN>
+  // |v|^p per segment: p = 1 and p = 2 without pow (as the distance kernels
+  // take d and d * d), any other p with pow.
+  template <int PK, typename TV>
+  __device__ __forceinline__ TV norm_term(TV a, TV p)
+  {
+    if (PK == 1)
+      return a;
+    if (PK == 2)
+      return a * a;
+    return pow(a, p);
+  }
+
+  template <typename TT, typename TV, int PK, bool KAHAN>
   __global__ __launch_bounds__(256) void norm_kernel(NormParams P)
   {
     using L = Lim<TT>;
@@ -24,16 +37,34 @@ namespace sbamd
     using Acc = typename std::conditional<KAHAN, KahanAcc, PlainAcc<TV>>::type;
     Acc acc;
     TT t = TT(0);
-    for (int64_t k = 0; k + 1 < m; ++k)
+    // Records come in batches of NORM_BATCH loads issued together, so a lane
+    // waits on memory once per batch, not once per record; the sum keeps the
+    // sequential order.
+    constexpr int NORM_BATCH = 8;
+    int64_t k = 0;
+    for (; k + NORM_BATCH < m; k += NORM_BATCH)
+    {
+      RecT x[NORM_BATCH];
+#pragma unroll
+      for (int u = 0; u < NORM_BATCH; ++u)
+        x[u] = r[k + u];
+#pragma unroll
+      for (int u = 0; u < NORM_BATCH; ++u)
+      {
+        acc.add(static_cast<TV>(x[u].t - t), norm_term<PK>(fabs(x[u].v), p));
+        t = x[u].t;
+      }
+    }
+    for (; k + 1 < m; ++k)
     {
       const RecT x = r[k];
-      const TV a = fabs(x.v);
-      acc.add(static_cast<TV>(x.t - t), L1 ? a : pow(a, p));
+      acc.add(static_cast<TV>(x.t - t), norm_term<PK>(fabs(x.v), p));
       t = x.t;
     }
-    const TV a = fabs(r[m - 1].v);
-    double res = acc.finish(L::max() - t, L1 ? a : pow(a, p));
-    if (!L1)
+    double res = acc.finish(L::max() - t, norm_term<PK>(fabs(r[m - 1].v), p));
+    if (PK == 2)
+      res = sqrt(res);
+    else if (PK != 1)
       res = pow(res, 1.0 / P.p);
     const int64_t o = P.perm[q];
     if (P.out_f64)
@@ -47,9 +78,11 @@ namespace sbamd
   {
     const unsigned grid = static_cast<unsigned>((prm.n + 255) / 256);
     if (prm.p == 1.0)
-      hipLaunchKernelGGL((norm_kernel<TT, TV, true, KAHAN>), dim3(grid), dim3(256), 0, stream, prm);
+      hipLaunchKernelGGL((norm_kernel<TT, TV, 1, KAHAN>), dim3(grid), dim3(256), 0, stream, prm);
+    else if (prm.p == 2.0)
+      hipLaunchKernelGGL((norm_kernel<TT, TV, 2, KAHAN>), dim3(grid), dim3(256), 0, stream, prm);
     else
-      hipLaunchKernelGGL((norm_kernel<TT, TV, false, KAHAN>), dim3(grid), dim3(256), 0, stream, prm);
+      hipLaunchKernelGGL((norm_kernel<TT, TV, 0, KAHAN>), dim3(grid), dim3(256), 0, stream, prm);
     SBAMD_HIP(hipGetLastError());
   }
 
