@@ -7,9 +7,12 @@
 // OffsetDataManager::init, offset_data_manager.hpp:42-68), which copies every
 // point on one host thread.
 //
-// A wavefront owns a PCF and walks its points 64 at a time; a point is kept
-// when the next point has a later time or it is the last one, and its record
-// index is the number of kept points before it (ballot + popcount). Inputs
+// A wavefront owns a PCF; a point is kept when the next point has a later
+// time or it is the last one, and its record index is the number of kept
+// points before it. Pass 1 only counts and walks 256 points per step, four
+// consecutive ones per lane with all of the lane's loads in flight together;
+// pass 2 walks 64 points per step, one per lane, so its record stores are
+// contiguous across the wavefront, and ranks with a ballot and mbcnt. Inputs
 // are strided views, so the same kernels read SoA arrays (stride = element
 // size) and AoS {t, v} point arrays (stride = 2 x element size); with
 // per-PCF byte bases they also read a file image as it lies, where each
@@ -23,6 +26,7 @@ namespace sbamd
   namespace
   {
     constexpr int PACK_WAVES = 4; // wavefronts (PCFs) per workgroup
+    constexpr int PACK_RUN = 4;   // consecutive points per lane and step
 
     template <typename T>
     __device__ __forceinline__ double load_at(const unsigned char* base, int64_t k, int64_t stride)
@@ -50,23 +54,33 @@ namespace sbamd
     const unsigned char* const tp = P.t + (P.pbase ? P.pbase[i] : 0);
     int64_t cnt = 0;
     bool bad = false;
-    for (int64_t c = b; c < e; c += 64)
+    // PACK_RUN consecutive points per lane and step: the lane's loads
+    // (its points and their two neighbours) are all in flight together
+    for (int64_t c = b; c < e; c += 64 * PACK_RUN)
     {
-      const int64_t k = c + lane;
-      const bool in = k < e;
-      bool keep = false;
-      if (in)
+      const int64_t k0 = c + PACK_RUN * lane;
+      double tt[PACK_RUN + 2]; // times of points k0 - 1 .. k0 + PACK_RUN
+#pragma unroll
+      for (int u = 0; u < PACK_RUN + 2; ++u)
       {
-        const double t = load_at<T>(tp, k, P.t_stride);
-        if (!isfinite(t))
-          bad = true;
-        if (k == b && t != 0.0)
-          bad = true;
-        if (k > b && t < load_at<T>(tp, k - 1, P.t_stride))
-          bad = true;
-        keep = k + 1 == e || t < load_at<T>(tp, k + 1, P.t_stride);
+        const int64_t k = k0 - 1 + u;
+        tt[u] = k >= b && k < e ? load_at<T>(tp, k, P.t_stride) : 0.0;
       }
-      cnt += __popcll(__ballot(keep));
+#pragma unroll
+      for (int u = 0; u < PACK_RUN; ++u)
+      {
+        const int64_t k = k0 + u;
+        const bool in = k < e;
+        const double t = tt[u + 1];
+        if (in && !isfinite(t))
+          bad = true;
+        if (in && k == b && t != 0.0)
+          bad = true;
+        if (in && k > b && t < tt[u])
+          bad = true;
+        const bool keep = in && (k + 1 == e || t < tt[u + 2]);
+        cnt += __popcll(__ballot(keep));
+      }
     }
     if (__ballot(bad) && lane == 0)
       atomicMin(P.bad, static_cast<unsigned long long>(i));
