@@ -20,7 +20,7 @@ namespace sbamd
   // 8 waves per SIMD (at most 64 VGPRs): the walk is latency-bound on its
   // dependent LDS loads, so residency matters more than registers. The
   // general-p operation carries an inlined pow, which would spill at 64 VGPRs.
-  __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OP == OPK_LP ? 4 : 8, 8))) void fwd_tile_kernel(FwdParams P)
+  __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OP == OPK_LP ? 4 : 6, 8))) void fwd_tile_kernel(FwdParams P)
   {
     extern __shared__ __align__(16) unsigned char smem[];
     using L = Lim<TT>;
@@ -95,6 +95,18 @@ namespace sbamd
         __syncthreads();
         const TT Tb = L::from_bits(S.tb[phase & 1]);
         phase_advance_cursor(S, P, Tb, tid);
+        __syncthreads();
+        // Steps of this lane's walk in the phase: every record of either PCF
+        // that ends before Tb is consumed, so the walk leaves each side's
+        // next-record index at that slot's new cursor. The wavefront runs the
+        // smallest count over its lanes under a scalar loop counter (no
+        // per-step time compare), then the rest under the compare.
+        int n_walk = 0x7fffffff;
+        if (valid)
+          n_walk = static_cast<int>(S.next[slot_f] + S.next[slot_g] - (lo_k + hi_k));
+        for (int o = 32; o > 0; o >>= 1)
+          n_walk = min(n_walk, __shfl_xor(n_walk, o));
+        const int n_uniform = __builtin_amdgcn_readfirstlane(n_walk);
 
         if (valid)
         {
@@ -120,7 +132,7 @@ namespace sbamd
           // Two steps per trip let registers alternate roles.
           auto step = [&]() {
             acc.add(len, pair_post<OP, TV>(x, p));
-            const RecT r = *lo_p;
+            const RecT r = lds_rec(lo_p);
             const RecT* np = lo_p + 1;
             const bool k = r.t <= hi_t;
             x = pair_pre<OP, TV>(r.v, hi_v);
@@ -133,6 +145,14 @@ namespace sbamd
             lo_p = nlp;
             hi_v = k ? hi_v : r.v;
           };
+          int n = 0;
+          for (; n + 2 <= n_uniform; n += 2)
+          {
+            step();
+            step();
+          }
+          if (n < n_uniform)
+            step();
           while (lo_t < Tb)
           {
             step();

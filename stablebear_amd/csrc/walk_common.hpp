@@ -204,6 +204,28 @@ namespace sbamd
     TV v;
   };
 
+  // One LDS read of a staged record. An FP64-time / FP32-value record is
+  // read as one vector load: field by field it becomes a 64-bit and a 32-bit
+  // LDS read, two LDS instructions per walk step instead of one.
+  template <typename TT, typename TV>
+  __device__ __forceinline__ Rec<TT, TV> lds_rec(const Rec<TT, TV>* p)
+  {
+    if constexpr (sizeof(TT) == 8 && sizeof(TV) == 4)
+    {
+      typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+      const u32x4 w = *reinterpret_cast<const u32x4*>(p);
+      // the lanes go through scalars first: this clang lowers
+      // __builtin_bit_cast of a vector element expression (w.z) to element 0
+      const uint32_t t_lo = w.x, t_hi = w.y, v_bits = w.z;
+      Rec<TT, TV> r;
+      r.t = __builtin_bit_cast(double, (static_cast<uint64_t>(t_hi) << 32) | t_lo);
+      r.v = __builtin_bit_cast(float, v_bits);
+      return r;
+    }
+    else
+      return *p;
+  }
+
   // Byte layout of the dynamic LDS block of a tile kernel:
   //   slot s (rows 0..R-1, then columns), record j at
   //     rows_off + (s * slot_recs(cap) + j) * REC
