@@ -415,15 +415,26 @@ namespace sbamd
     const int cap = P.cap;
     const RecT* __restrict__ g_r = static_cast<const RecT*>(P.r_rec);
     const RecT* __restrict__ g_c = static_cast<const RecT*>(P.c_rec);
-    if (tid < NS)
+    if ((tid & ~63) < NS)
     {
-      const int64_t c = S.next[tid];
-      S.cur[tid] = c;
-      const int64_t rem = S.m[tid] - c;
-      TT tbs = L::inf();
-      if (rem > cap)
-        tbs = (tid < P.R ? g_r : g_c)[S.base[tid] + c + cap - 1].t;
-      atomicMin(&S.tb[phase & 1], L::bits(tbs));
+      // the proposals of a wavefront's slot threads are reduced in registers
+      // (times are >= 0, so their bit patterns order as the times do) and
+      // entered with one LDS atomic per wavefront
+      typename L::Bits tbs = L::bits(L::inf());
+      if (tid < NS)
+      {
+        const int64_t c = S.next[tid];
+        S.cur[tid] = c;
+        if (S.m[tid] - c > cap)
+          tbs = L::bits((tid < P.R ? g_r : g_c)[S.base[tid] + c + cap - 1].t);
+      }
+      for (int o = 32; o > 0; o >>= 1)
+      {
+        const typename L::Bits u = __shfl_xor(tbs, o);
+        tbs = u < tbs ? u : tbs;
+      }
+      if ((tid & 63) == 0)
+        atomicMin(&S.tb[phase & 1], tbs);
     }
     const int lane = tid & 63;
     const int nwaves = nthreads >> 6;
