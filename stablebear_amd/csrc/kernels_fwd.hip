@@ -101,12 +101,10 @@ namespace sbamd
         // next-record index at that slot's new cursor. The wavefront runs the
         // smallest count over its lanes under a scalar loop counter (no
         // per-step time compare), then the rest under the compare.
-        int n_walk = 0x7fffffff;
+        uint32_t n_walk = 0x7fffffffu;
         if (valid)
-          n_walk = static_cast<int>(S.next[slot_f] + S.next[slot_g] - (lo_k + hi_k));
-        for (int o = 32; o > 0; o >>= 1)
-          n_walk = min(n_walk, __shfl_xor(n_walk, o));
-        const int n_uniform = __builtin_amdgcn_readfirstlane(n_walk);
+          n_walk = static_cast<uint32_t>(S.next[slot_f] + S.next[slot_g] - (lo_k + hi_k));
+        const int n_uniform = static_cast<int>(wave_min(n_walk));
 
         if (valid)
         {

@@ -398,6 +398,61 @@ namespace sbamd
     return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
   }
 
+  // Minimum over the 64 lanes of a wavefront (all lanes active), as a
+  // wavefront-uniform value. Within each row of 16 lanes, shifts right by 1,
+  // 2, 4 and 8 (DPP row_shr; a lane without a source keeps its own value)
+  // leave the row's minimum in the row's last lane; the four row minima are
+  // read from lanes 15, 31, 47 and 63. No LDS round trips.
+  template <int SHIFT>
+  __device__ __forceinline__ uint32_t dpp_shr(uint32_t v)
+  {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(static_cast<int>(v), static_cast<int>(v), 0x110 + SHIFT, 0xf, 0xf, false));
+  }
+
+  __device__ __forceinline__ uint32_t wave_min(uint32_t v)
+  {
+    v = min(v, dpp_shr<1>(v));
+    v = min(v, dpp_shr<2>(v));
+    v = min(v, dpp_shr<4>(v));
+    v = min(v, dpp_shr<8>(v));
+    const uint32_t a = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 15));
+    const uint32_t b = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 31));
+    const uint32_t c = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 47));
+    const uint32_t d = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+    return min(min(a, b), min(c, d));
+  }
+
+  template <int SHIFT>
+  __device__ __forceinline__ unsigned long long dpp_shr(unsigned long long v)
+  {
+    const uint32_t lo = dpp_shr<SHIFT>(static_cast<uint32_t>(v));
+    const uint32_t hi = dpp_shr<SHIFT>(static_cast<uint32_t>(v >> 32));
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+  }
+
+  __device__ __forceinline__ unsigned long long wave_min(unsigned long long v)
+  {
+    unsigned long long u = dpp_shr<1>(v);
+    v = u < v ? u : v;
+    u = dpp_shr<2>(v);
+    v = u < v ? u : v;
+    u = dpp_shr<4>(v);
+    v = u < v ? u : v;
+    u = dpp_shr<8>(v);
+    v = u < v ? u : v;
+    unsigned long long r = ~0ull;
+#pragma unroll
+    for (int row = 0; row < 4; ++row)
+    {
+      const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 16 * row + 15));
+      const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), 16 * row + 15));
+      const unsigned long long x = (static_cast<unsigned long long>(hi) << 32) | lo;
+      r = x < r ? x : r;
+    }
+    return r;
+  }
+
   // First half of a phase (before the barrier): each slot proposes the phase
   // end (start time of its (cap+1)-th unconsumed record), and the workgroup
   // copies up to cap records of every slot into LDS, one wavefront per slot
@@ -428,11 +483,7 @@ namespace sbamd
         if (S.m[tid] - c > cap)
           tbs = L::bits((tid < P.R ? g_r : g_c)[S.base[tid] + c + cap - 1].t);
       }
-      for (int o = 32; o > 0; o >>= 1)
-      {
-        const typename L::Bits u = __shfl_xor(tbs, o);
-        tbs = u < tbs ? u : tbs;
-      }
+      tbs = wave_min(tbs);
       if ((tid & 63) == 0)
         atomicMin(&S.tb[phase & 1], tbs);
     }
