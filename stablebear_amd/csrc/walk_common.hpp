@@ -489,30 +489,44 @@ namespace sbamd
     }
     const int lane = tid & 63;
     const int nwaves = nthreads >> 6;
-    for (int s = tid >> 6; s < NS; s += nwaves)
+    if constexpr (sizeof(RecT) == 16)
     {
-      const int64_t m = wave_uniform(S.m[s]);
-      const int64_t first = wave_uniform(S.next[s]) - LEAD; // window record 0
-      // window records j in [jlo, jhi): record first + j exists
-      const int jlo = first < 0 ? 1 : 0;
-      const int jhi = static_cast<int>(m - first < cap + LEAD ? m - first : cap + LEAD);
-      const RecT* src = (s < P.R ? g_r : g_c) + wave_uniform(S.base[s]);
-      const uint32_t dst = wave_uniform(S.lbase[s]);
-      if constexpr (sizeof(RecT) == 16)
+      // Direct global -> LDS loads (global_load_lds_dwordx4): lane i of a
+      // 64-record chunk lands at chunk base + 16 i, which is exactly the
+      // window layout, so records bypass registers and every slot's loads
+      // are in flight together; one vmcnt wait below covers them. The slot
+      // tables are read per lane (one broadcast LDS read each) and the
+      // window address comes from the slot index, so a slot costs a few
+      // vector operations and no readfirstlane round trips.
+      const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+      const uint32_t slot_bytes = static_cast<uint32_t>(slot_recs(cap)) * 16u;
+      for (int s = w; s < NS; s += nwaves)
       {
-        // Direct global -> LDS loads (global_load_lds_dwordx4): lane i of a
-        // 64-record chunk lands at chunk base + 16 i, which is exactly the
-        // window layout, so records bypass registers and every slot's loads
-        // are in flight together; one vmcnt wait below covers them.
-        for (int c0 = 0; c0 < jhi; c0 += 64)
+        const int64_t m = S.m[s];
+        const int64_t first = S.next[s] - LEAD; // window record 0
+        // window records j in [jlo, jhi): record first + j exists
+        const int jlo = first < 0 ? 1 : 0;
+        const int jhi = static_cast<int>(m - first < cap + LEAD ? m - first : cap + LEAD);
+        const RecT* src = (s < P.R ? g_r : g_c) + (S.base[s] + first);
+        const uint32_t dst = static_cast<uint32_t>(S.lay.rows_off) + static_cast<uint32_t>(s) * slot_bytes;
+        for (int c0 = 0; c0 < cap + LEAD; c0 += 64)
         {
           const int j = c0 + lane;
           if (j >= jlo && j < jhi)
-            lds_dma16(src + (first + j), S.smem + dst + 16u * static_cast<uint32_t>(c0));
+            lds_dma16(src + j, S.smem + dst + 16u * static_cast<uint32_t>(c0));
         }
       }
-      else
+    }
+    else
+    {
+      for (int s = tid >> 6; s < NS; s += nwaves)
       {
+        const int64_t m = wave_uniform(S.m[s]);
+        const int64_t first = wave_uniform(S.next[s]) - LEAD; // window record 0
+        const int jlo = first < 0 ? 1 : 0;
+        const int jhi = static_cast<int>(m - first < cap + LEAD ? m - first : cap + LEAD);
+        const RecT* src = (s < P.R ? g_r : g_c) + wave_uniform(S.base[s]);
+        const uint32_t dst = wave_uniform(S.lbase[s]);
         for (int j = lane; j < jhi; j += 64)
           if (j >= jlo)
             S.rec(dst + static_cast<uint32_t>(j) * static_cast<uint32_t>(sizeof(RecT))) = src[first + j];
