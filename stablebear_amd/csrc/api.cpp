@@ -1341,8 +1341,6 @@ extern "C"
       if (s->records > INT32_MAX)
         fail(SBAMD_ERR_INVALID, "backward supports sets of up to 2^31 - 1 records");
       const size_t gb = grad_dtype == SBAMD_F64 ? 8 : 4;
-      launch_fill_zero(grad_v_device, static_cast<size_t>(s->points) * gb, st);
-      launch_fill_zero(grad_t_device, static_cast<size_t>(s->points) * gb, st);
       const size_t M = static_cast<size_t>(s->records);
       const size_t N = static_cast<size_t>(s->n);
       if (plan->bwd_q.bytes < M * 8)
@@ -1368,9 +1366,14 @@ extern "C"
         plan->bwd_a = DeviceBuffer(M * 8);
         plan->bwd_w = DeviceBuffer(N * 8);
       }
-      launch_fill_zero(plan->bwd_q.ptr, M * 8, st);
-      launch_fill_zero(plan->bwd_a.ptr, M * 8, st);
-      launch_fill_zero(plan->bwd_w.ptr, N * 8, st);
+      {
+        // the gradients and the backward's accumulators, cleared by one launch
+        void* const ptrs[5] = {grad_v_device, grad_t_device, plan->bwd_q.ptr, plan->bwd_a.ptr,
+                               plan->bwd_w.ptr};
+        const size_t bytes[5] = {static_cast<size_t>(s->points) * gb, static_cast<size_t>(s->points) * gb,
+                                 M * 8, M * 8, N * 8};
+        launch_fill_zero_multi(ptrs, bytes, 5, st);
+      }
       BwdParams B;
       std::memset(&B, 0, sizeof(B));
       B.n = s->n;

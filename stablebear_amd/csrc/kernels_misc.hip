@@ -153,4 +153,48 @@ namespace sbamd
                        static_cast<uint32_t*>(ptr), words);
     SBAMD_HIP(hipGetLastError());
   }
+
+  // Several buffers cleared by one launch (a fill kernel, not a memset, so
+  // that a captured graph replays it correctly).
+  struct FillList
+  {
+    uint32_t* ptr[FILL_MAX];
+    size_t words[FILL_MAX];
+    int n;
+  };
+
+  __global__ void fill_zero_multi_kernel(FillList L)
+  {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    const size_t i0 = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (int b = 0; b < L.n; ++b)
+      for (size_t i = i0; i < L.words[b]; i += stride)
+        L.ptr[b][i] = 0u;
+  }
+
+  void launch_fill_zero_multi(void* const* ptrs, const size_t* bytes, int n, hipStream_t stream)
+  {
+    if (n < 0 || n > FILL_MAX)
+      fail(SBAMD_ERR_INVALID, "too many buffers for one fill");
+    FillList L{};
+    size_t most = 0;
+    for (int b = 0; b < n; ++b)
+    {
+      if (bytes[b] % 4 != 0)
+        fail(SBAMD_ERR_INVALID, "fill size must be a multiple of 4 bytes");
+      if (bytes[b] == 0)
+        continue;
+      L.ptr[L.n] = static_cast<uint32_t*>(ptrs[b]);
+      L.words[L.n] = bytes[b] / 4;
+      most = L.words[L.n] > most ? L.words[L.n] : most;
+      ++L.n;
+    }
+    if (L.n == 0)
+      return;
+    size_t blocks = (most + 255) / 256;
+    if (blocks > 65536)
+      blocks = 65536;
+    hipLaunchKernelGGL(fill_zero_multi_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, L);
+    SBAMD_HIP(hipGetLastError());
+  }
 }

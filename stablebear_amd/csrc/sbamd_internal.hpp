@@ -274,6 +274,8 @@ namespace sbamd
   void launch_norm(const NormParams& prm, int tt_f64, int tv_f64, hipStream_t stream);
   void launch_backward(const BwdParams& prm, hipStream_t stream);
   void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream);
+  constexpr int FILL_MAX = 8;
+  void launch_fill_zero_multi(void* const* ptrs, const size_t* bytes, int n, hipStream_t stream);
 
   // Device-side packing of a strided CSR view (kernels_pack.hip).
   struct PackParams
