@@ -220,6 +220,8 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 kernel leg")
     ap.add_argument("--no-ops", action="store_true", help="skip the L2 / kernel legs")
     ap.add_argument("--autotune", action="store_true", help="autotune tile shapes first")
+    ap.add_argument("--tile", type=str, default=None,
+                    help="tile shape rows,cols,cap of the L1 pdist leg (default: the plan's)")
     ap.add_argument("--only-fwdbwd", action="store_true",
                     help="run only the fwd+bwd leg (for profiling the backward)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -292,7 +294,8 @@ def main(argv=None):
     results = {}
     for prec in ("fp32c", "fp64"):
         ds = runtime.DeviceSet(csr, prec)
-        plan = runtime.Plan(ds, op="l1")
+        shape = tuple(int(x) for x in args.tile.split(",")) if args.tile else None
+        plan = runtime.Plan(ds, op="l1", shape=shape)
         f64 = prec == "fp64"
         out = torch.empty(plan.out_count, dtype=torch.float64 if f64 else torch.float32,
                           device="cuda")
@@ -321,8 +324,8 @@ def main(argv=None):
     achieved = flops / (ms * 1e-3) / 1e12
     in_bytes = int(off[-1]) * (8 + 4) + 8 * (n + 1)
     alg_bytes = in_bytes + pairs * 4
-    roofline = {"bound": "mfma", "compute_unit": "valu (no MFMA: the walk is compare/select/"
-                "FMA work on the vector ALUs; peak = MI355X FP32 vector rate)",
+    roofline = {"bound": "valu", "compute_unit": "vector ALU (no MFMA: the walk is compare/"
+                "select/FMA work on the vector ALUs; peak = MI355X FP32 vector rate)",
                 "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
                 "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": _traffic(),
                 "flops_per_segment": FLOPS_PER_SEGMENT["l1"], "segments": plan.segments,

@@ -96,15 +96,17 @@ typedef struct
   int32_t location;        /* SBAMD_HOST or SBAMD_DEVICE */
 } sbamd_pcf_view;
 
-/* Pair-tile shape of the lane-per-pair kernel: rows x cols pairs per
- * workgroup (rows * cols <= 256), `cap` breakpoint records per PCF staged in
- * LDS per phase. Zero fields mean "use the tuned or default value". */
+/* Pair-tile shape of the tile kernel: rows x cols pairs per workgroup
+ * (rows + cols <= 64), `cap` breakpoint records per PCF staged in LDS per
+ * phase, `lane_pairs` pairs walked by each lane (1 or 2; a workgroup has
+ * rows * ceil(cols / lane_pairs) lanes, at most 512). Zero fields mean "use
+ * the tuned or default value". */
 typedef struct
 {
   int32_t rows;
   int32_t cols;
   int32_t cap;
-  int32_t reserved;
+  int32_t lane_pairs;
 } sbamd_tile_shape;
 
 typedef struct
@@ -117,7 +119,7 @@ typedef struct
   int32_t rows, cols, cap; /* tile shape in use */
   int32_t problems;        /* 1, or k for a batched plan */
   int32_t long_pairs;      /* 1 if the plan runs the long-pair kernel */
-  int32_t reserved;
+  int32_t lane_pairs;      /* pairs per lane of the tile kernel */
 } sbamd_plan_stats;
 
 /* ---- status / device ---- */
@@ -244,7 +246,7 @@ int sbamd_graph_destroy(sbamd_graph* graph);
 /* ---- autotuner: times candidate tile shapes on this plan and keeps the best
  * (also returned). Candidates are limited by the LDS budget. For FP32C
  * plans (or plans set to long-pair mode 1) the long-pair kernel is a
- * candidate too; best->reserved is 1 when it won (the plan then uses it). ---- */
+ * candidate too; when it wins the plan uses it (sbamd_plan_info long_pairs). ---- */
 int sbamd_autotune(sbamd_plan* plan, void* out_device, int32_t out_dtype, int32_t reps,
                    sbamd_tile_shape* best, double* best_ms);
 int sbamd_plan_set_shape(sbamd_plan* plan, const sbamd_tile_shape* shape);

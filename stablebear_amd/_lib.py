@@ -71,7 +71,7 @@ class IoInfo(ctypes.Structure):
 
 class TileShape(ctypes.Structure):
     _fields_ = [("rows", ctypes.c_int32), ("cols", ctypes.c_int32), ("cap", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("lane_pairs", ctypes.c_int32)]
 
 
 class PlanStats(ctypes.Structure):
@@ -79,7 +79,7 @@ class PlanStats(ctypes.Structure):
                 ("out_count", ctypes.c_uint64), ("tiles", ctypes.c_uint64),
                 ("input_bytes", ctypes.c_uint64), ("rows", ctypes.c_int32),
                 ("cols", ctypes.c_int32), ("cap", ctypes.c_int32), ("problems", ctypes.c_int32),
-                ("long_pairs", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("long_pairs", ctypes.c_int32), ("lane_pairs", ctypes.c_int32)]
 
 
 # name -> (restype, argtypes); restype None means int status

@@ -141,13 +141,11 @@ namespace sbamd
 {
   namespace
   {
-    void check_shape(int R, int C, int cap)
+    void check_shape(int R, int C, int cap, int ppl)
     {
-      if (R < 1 || C < 1 || R * C > 256 || cap < 2 || cap > 4096)
-        fail(SBAMD_ERR_INVALID, "tile shape out of range (rows*cols <= 256, 2 <= cap <= 4096)");
-      const int block = std::max(64, (R * C + 63) / 64 * 64);
-      if (R + C > block)
-        fail(SBAMD_ERR_INVALID, "tile shape needs rows + cols <= workgroup size");
+      if (!tile_shape_ok(R, C, cap, ppl))
+        fail(SBAMD_ERR_INVALID, "tile shape out of range (rows + cols <= 64, lane_pairs 1 or 2, "
+                                "at most 512 lanes, 2 <= cap <= 4096)");
     }
 
     void set_layout_for(int32_t precision, int32_t in_dtype, int32_t& tt_f64, int32_t& tv_f64)
@@ -528,6 +526,7 @@ namespace sbamd
       P.R = pl.R;
       P.C = pl.C;
       P.cap = pl.cap;
+      P.ppl = pl.ppl;
       P.stride = pl.R + pl.C;
       P.symmetric = pl.cdist ? 0 : 1;
       P.include_diag = pl.out_kind == SBAMD_OUT_SYMMAT ? 1 : 0;
@@ -1032,10 +1031,12 @@ extern "C"
           pl->C = shape->cols;
         if (shape->cap)
           pl->cap = shape->cap;
+        if (shape->lane_pairs)
+          pl->ppl = shape->lane_pairs;
       }
       if (!pl->cdist)
         pl->C = pl->R;
-      check_shape(pl->R, pl->C, pl->cap);
+      check_shape(pl->R, pl->C, pl->cap, pl->ppl);
       pl->rebuild_tiles();
       decide_long(*pl);
       *out = pl.release();
@@ -1122,9 +1123,11 @@ extern "C"
           pl->R = shape->rows;
         if (shape->cap)
           pl->cap = shape->cap;
+        if (shape->lane_pairs)
+          pl->ppl = shape->lane_pairs;
       }
       pl->C = pl->R;
-      check_shape(pl->R, pl->C, pl->cap);
+      check_shape(pl->R, pl->C, pl->cap, pl->ppl);
       pl->rebuild_tiles();
       *out = pl.release();
     });
@@ -1155,7 +1158,7 @@ extern "C"
       info->cap = plan->cap;
       info->problems = static_cast<int32_t>(plan->h_probs.size());
       info->long_pairs = plan->use_long ? 1 : 0;
-      info->reserved = 0;
+      info->lane_pairs = plan->ppl;
     });
   }
 
@@ -1210,12 +1213,14 @@ extern "C"
       int R = shape->rows ? shape->rows : plan->R;
       int C = shape->cols ? shape->cols : plan->C;
       int cap = shape->cap ? shape->cap : plan->cap;
+      int ppl = shape->lane_pairs ? shape->lane_pairs : plan->ppl;
       if (!plan->cdist)
         C = R;
-      check_shape(R, C, cap);
+      check_shape(R, C, cap, ppl);
       plan->R = R;
       plan->C = C;
       plan->cap = cap;
+      plan->ppl = ppl;
       plan->rebuild_tiles();
     });
   }
@@ -1486,31 +1491,31 @@ extern "C"
         reps = 1;
       use_device(plan->ctx);
       hipStream_t st = plan->ctx->stream;
-      const int R0 = plan->R, C0 = plan->C, cap0 = plan->cap;
+      const int R0 = plan->R, C0 = plan->C, cap0 = plan->cap, ppl0 = plan->ppl;
       plan->use_long = false; // tile candidates first
       struct Cand
       {
-        int R, C, cap;
+        int R, C, cap, ppl;
       };
       std::vector<Cand> cands;
-      // 16 x 16 tiles (256 lanes) with LDS windows small enough for several
-      // resident workgroups; 8 x 8 only where a plan is too small to fill
-      // the device with 16 x 16 tiles.
-      const int sides[] = {16, 8};
-      const int caps[] = {24, 32, 48, 64, 96};
-      const size_t budget = 48 * 1024;
-      uint64_t big_tiles = 0; // 16 x 16 tiles of the plan
+      // 32 x 32 and 16 x 16 tiles with LDS windows that leave room for two
+      // or more resident workgroups per CU; 8 x 8 only where a plan is too
+      // small to fill the device with 16 x 16 tiles.
+      const int sides[] = {32, 16, 8};
+      const int caps[] = {32, 48, 64, 96};
+      const size_t budget = std::min<size_t>(plan->ctx->lds_per_block, 80 * 1024);
+      uint64_t mid_tiles = 0; // 16 x 16 tiles of the plan
       for (size_t b = 0; b < plan->sets.size(); ++b)
       {
         const uint64_t nr = (static_cast<uint64_t>(plan->sets[b]->n) + 15) / 16;
         if (plan->cdist)
         {
-          big_tiles = nr * ((static_cast<uint64_t>(plan->sets[1]->n) + 15) / 16);
+          mid_tiles = nr * ((static_cast<uint64_t>(plan->sets[1]->n) + 15) / 16);
           break;
         }
-        big_tiles += nr * (nr + 1) / 2;
+        mid_tiles += nr * (nr + 1) / 2;
       }
-      const bool small = big_tiles < 4ull * static_cast<uint64_t>(plan->ctx->cu_count);
+      const bool small = mid_tiles < 4ull * static_cast<uint64_t>(plan->ctx->cu_count);
       for (int R : sides)
         for (int C : sides)
         {
@@ -1518,18 +1523,23 @@ extern "C"
             continue;
           if ((R < 16 || C < 16) && !small)
             continue;
-          for (int cap : caps)
+          for (int ppl = 1; ppl <= 2; ++ppl)
           {
-            if (fwd_lds_bytes(plan->tt_f64, plan->tv_f64, R, C, cap) > budget)
+            if (!tile_shape_ok(R, C, 2, ppl))
               continue;
-            cands.push_back({R, C, cap});
+            for (int cap : caps)
+            {
+              if (fwd_lds_bytes(plan->tt_f64, plan->tv_f64, R, C, cap) > budget)
+                continue;
+              cands.push_back({R, C, cap, ppl});
+            }
           }
         }
       hipEvent_t e0, e1;
       SBAMD_HIP(hipEventCreate(&e0));
       SBAMD_HIP(hipEventCreate(&e1));
       double bestv = std::numeric_limits<double>::infinity();
-      Cand bc{R0, C0, cap0};
+      Cand bc{R0, C0, cap0, ppl0};
       try
       {
         for (const Cand& c : cands)
@@ -1537,6 +1547,7 @@ extern "C"
           plan->R = c.R;
           plan->C = c.C;
           plan->cap = c.cap;
+          plan->ppl = c.ppl;
           plan->rebuild_tiles();
           execute_range(*plan, out_device, out_dtype, nullptr, 0,
                         work_units(*plan), st); // warm-up
@@ -1565,6 +1576,7 @@ extern "C"
       plan->R = bc.R;
       plan->C = bc.C;
       plan->cap = bc.cap;
+      plan->ppl = bc.ppl;
       plan->rebuild_tiles();
       // the long-pair kernel is a candidate too (FP32C plans, or plans that
       // asked for it)
@@ -1606,7 +1618,7 @@ extern "C"
         best->rows = bc.R;
         best->cols = bc.C;
         best->cap = bc.cap;
-        best->reserved = chose_long ? 1 : 0;
+        best->lane_pairs = bc.ppl;
       }
       if (best_ms)
         *best_ms = bestv;
@@ -1937,12 +1949,12 @@ extern "C"
         std::vector<int64_t> mcs(mc.size());
         for (size_t q = 0; q < pc.size(); ++q)
           mcs[q] = mc[pc[q]];
-        check_shape(tile_rows, tile_cols, 64);
+        check_shape(tile_rows, tile_cols, 64, 2);
         build_tiles(mrs, &mcs, tile_rows, tile_cols, false, 0, std::numeric_limits<int64_t>::max(), out);
       }
       else
       {
-        check_shape(tile_rows, tile_rows, 64);
+        check_shape(tile_rows, tile_rows, 64, 2);
         build_tiles(mrs, nullptr, tile_rows, tile_rows, false, 0, std::numeric_limits<int64_t>::max(), out);
       }
       *n_tiles = static_cast<int64_t>(out.size());

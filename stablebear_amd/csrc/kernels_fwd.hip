@@ -1,4 +1,4 @@
-// Forward pairwise integration: one workgroup per pair tile, one lane per pair.
+// Forward pairwise integration: one workgroup per pair tile, two pairs per lane.
 //
 // Replaces the reference's one-thread-per-pair kernel over global memory
 // (include/sbear/cuda/cuda_pcf_kernel.cuh:36-136). Here a workgroup owns an
@@ -6,33 +6,42 @@
 // wave walk pairs of near-equal length), and stages the R + C PCFs of the tile
 // in LDS in time phases: every staged record serves all R (or C) pairs that
 // use it. A phase ends at the time Tb where the first tile PCF would overflow
-// `cap` staged records; each lane walks its pair up to Tb, then the next
+// `cap` staged records; each lane walks its pairs up to Tb, then the next
 // records are staged.
 //
 // The walk step is branch-free: one compare picks the side whose record ends
 // first, one LDS load fetches that side's next record, and selects update the
 // state, so a wavefront issues the same instruction stream for every lane.
+// Each step waits on its own LDS load, so a lane walks two pairs, (r, c) and
+// (r, c + ceil(C/2)), and issues both pairs' loads before it consumes either: two
+// independent chains per lane keep the SIMDs issuing while loads are in
+// flight.
 #include "walk_common.hpp"
 
 namespace sbamd
 {
-  template <typename TT, typename TV, int OP, bool KAHAN>
-  // 8 waves per SIMD (at most 64 VGPRs): the walk is latency-bound on its
-  // dependent LDS loads, so residency matters more than registers. The
-  // general-p operation carries an inlined pow, which would spill at 64 VGPRs.
-  __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OP == OPK_LP ? 4 : 6, 8))) void fwd_tile_kernel(FwdParams P)
+  // Register budgets: one pair per lane fits 80 VGPRs (6 waves per SIMD);
+  // two pairs per lane need about 100 (4 waves per SIMD, so two 32 x 32
+  // workgroups of 8 waves per CU). The general-p operation carries an
+  // inlined pow and gets twice the registers.
+  template <int OP, int NCH>
+  constexpr int fwd_min_waves() { return (OP == OPK_LP ? 2 : 4) + (NCH == 1 ? 2 : 0); }
+
+  template <typename TT, typename TV, int OP, bool KAHAN, int NCH>
+  __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(fwd_min_waves<OP, NCH>(), 8))) void fwd_tile_kernel(FwdParams P)
   {
     extern __shared__ __align__(16) unsigned char smem[];
     using L = Lim<TT>;
     using RecT = Rec<TT, TV>;
+    constexpr uint32_t RB = sizeof(RecT);
 
-    const LdsLayout lay = lds_layout(sizeof(TT), P.R, P.C, P.cap, 0);
+    const LdsLayout lay = lds_layout(sizeof(TT), P.R, P.C, P.cap);
     const TileLds<TT, TV> S(smem, lay);
-    constexpr uint32_t SF = sizeof(RecT); // record stride within a slot
 
     const int tid = threadIdx.x;
     const int nthreads = blockDim.x;
     const TV p = static_cast<TV>(P.p);
+    const int half = tile_lane_cols(P.C, NCH);
 
     // Tiles come either one per workgroup (grid = tiles, the dispatcher hands
     // them out in launch order) or, for persistent workgroups, from an atomic
@@ -40,7 +49,7 @@ namespace sbamd
     uint32_t* const s_next_tile = reinterpret_cast<uint32_t*>(smem + lay.misc_off);
     // row sums (when requested): the tile's R partial sums, one global atomic
     // per row and tile instead of one per pair
-    double* const s_rows = reinterpret_cast<double*>(smem + lay.total);
+    double* const s_rows = reinterpret_cast<double*>(smem + lay.rows_off);
     int64_t ti = blockIdx.x;
     for (;; ti += gridDim.x)
     {
@@ -60,137 +69,201 @@ namespace sbamd
       tile_begin(S, P, tl, pr, tid);
       if (P.row_sum && tid < P.R)
         s_rows[tid] = 0.0;
-      __syncthreads();
 
-      // Pair of this lane (sorted positions a = r0 + lr, b = c0 + lc).
-      const int lr = tid / P.C;
-      const int lc = tid - lr * P.C;
-      bool valid = tid < P.R * P.C && lr < tl.nr && lc < tl.nc;
-      if (valid && (tl.flags & TILE_DIAG))
-        valid = P.include_diag ? (lr >= lc) : (lr > lc);
-      const int slot_f = valid ? lr : 0;
-      const int slot_g = valid ? P.R + lc : P.R;
+      // The lane's pairs: row lr against columns lc + q ceil(C / NCH). A pair
+      // outside the tile (edge tiles, the upper half of diagonal tiles) walks
+      // the pair of slots 0 and R instead, which every tile has, and stores
+      // nothing.
+      const int lr = tid / half;
+      const int lc = tid - lr * half;
+      bool valid[NCH];
+      int slot_f[NCH], slot_g[NCH];
+      uint32_t wb_f[NCH], wb_g[NCH];
+#pragma unroll
+      for (int q = 0; q < NCH; ++q)
+      {
+        const int c = lc + q * half;
+        bool v = lr < tl.nr && c < tl.nc;
+        if (v && (tl.flags & TILE_DIAG))
+          v = P.include_diag ? (lr >= c) : (lr > c);
+        valid[q] = v;
+        slot_f[q] = v ? lr : 0;
+        slot_g[q] = v ? P.R + c : P.R;
+        wb_f[q] = S.win(slot_f[q]);
+        wb_g[q] = S.win(slot_g[q]);
+      }
 
-      // The two current records of the pair, ordered by end time: the next
-      // union breakpoint is always lo.t, and the side that owns lo advances.
-      // (Which PCF lo belongs to does not matter to a symmetric pair op.)
-      // lo_p / hi_p: LDS address of the owning PCF's next record; between
-      // phases they are kept as (slot, record index) instead. x is the first
-      // half of the pair op of the current segment (pair_pre); lo's value is
-      // not kept, since after a step the two current values are {new
-      // record's value, hi_v} whichever side advanced, and every pair op is
-      // symmetric in its operands. len is the length of the current segment
-      // [previous breakpoint, lo_t), formed in the time type when the
-      // segment starts and rounded to the value type.
-      TT lo_t = L::inf(), hi_t = L::inf();
-      TV len = TV(0), x = TV(0), hi_v = TV(0);
-      int lo_s = slot_f, hi_s = slot_g;  // owning slots
-      int64_t lo_k = 1, hi_k = 1;        // record index of each owner's next record
+      // Per pair: the two current records ordered by end time. The next union
+      // breakpoint is always lo.t, and the side that owns lo advances (which
+      // PCF lo belongs to does not matter to a symmetric pair op). lo_p /
+      // hi_p: LDS address of the owning PCF's next record. At the end of a
+      // phase both point at their slot's cursor, which becomes record 0 of the
+      // slot's next window, so between phases a pair only keeps which side
+      // owns lo. x is the first half of the pair op of the current segment
+      // (pair_pre); lo's value is not kept, since after a step the two current
+      // values are {new record's value, hi_v} whichever side advanced, and
+      // every pair op is symmetric in its operands. len is the length of the
+      // current segment [previous breakpoint, lo_t), formed in the time type
+      // when the segment starts and rounded to the value type.
+      TT lo_t[NCH], hi_t[NCH];
+      TV len[NCH], x[NCH], hi_v[NCH];
+      uint32_t lo_p[NCH], hi_p[NCH];
+      bool lo_g[NCH];
       using Acc = typename std::conditional<KAHAN, KahanAcc, PlainAcc<TV>>::type;
-      Acc acc;
+      Acc acc[NCH];
+#pragma unroll
+      for (int q = 0; q < NCH; ++q)
+      {
+        lo_t[q] = hi_t[q] = L::inf();
+        len[q] = x[q] = hi_v[q] = TV(0);
+        lo_p[q] = hi_p[q] = 0;
+        lo_g[q] = false;
+      }
+      __syncthreads();
 
       for (int phase = 0;; ++phase)
       {
-        phase_stage(S, P, phase, tid, nthreads);
+        phase_stage(S, P, tid, nthreads);
         __syncthreads();
-        const TT Tb = L::from_bits(S.tb[phase & 1]);
-        phase_advance_cursor(S, P, Tb, tid);
+        const TT Tb = L::from_bits(*S.tb);
+        phase_cursors(S, P, Tb, tid);
         __syncthreads();
-        // Steps of this lane's walk in the phase: every record of either PCF
-        // that ends before Tb is consumed, so the walk leaves each side's
-        // next-record index at that slot's new cursor. The wavefront runs the
-        // smallest count over its lanes under a scalar loop counter (no
-        // per-step time compare), then the rest under the compare.
-        uint32_t n_walk = 0x7fffffffu;
-        if (valid)
-          n_walk = static_cast<uint32_t>(S.next[slot_f] + S.next[slot_g] - (lo_k + hi_k));
-        const int n_uniform = static_cast<int>(wave_min(n_walk));
 
-        if (valid)
+        // Steps of each pair in the phase: every record of either PCF that
+        // starts before Tb is consumed (the slots' cursor counts); in the
+        // first phase the two first records are read before the walk. The
+        // wavefront runs the smallest count over its lanes' pairs with both
+        // pairs' loads in flight, then the rest under the time compare.
+        uint32_t n_min = 0x7fffffffu;
+        uint32_t n_steps[NCH];
+#pragma unroll
+        for (int q = 0; q < NCH; ++q)
         {
+          const uint32_t n = static_cast<uint32_t>(S.cnt[slot_f[q]] + S.cnt[slot_g[q]] - (phase == 0 ? 2 : 0));
+          n_steps[q] = n;
+          n_min = n < n_min ? n : n_min;
           if (phase == 0)
           {
-            const RecT rf = S.rec(S.lbase[slot_f]);
-            const RecT rg = S.rec(S.lbase[slot_g]);
+            const RecT rf = lds_rec(&S.rec(wb_f[q]));
+            const RecT rg = lds_rec(&S.rec(wb_g[q]));
             const bool k = rf.t <= rg.t;
-            lo_t = k ? rf.t : rg.t;
-            hi_t = k ? rg.t : rf.t;
-            hi_v = k ? rg.v : rf.v;
-            x = pair_pre<OP, TV>(rf.v, rg.v);
-            len = static_cast<TV>(lo_t - TT(0));
-            lo_s = k ? slot_f : slot_g;
-            hi_s = k ? slot_g : slot_f;
+            lo_t[q] = k ? rf.t : rg.t;
+            hi_t[q] = k ? rg.t : rf.t;
+            hi_v[q] = k ? rg.v : rf.v;
+            x[q] = pair_pre<OP, TV>(rf.v, rg.v);
+            len[q] = static_cast<TV>(lo_t[q] - TT(0));
+            lo_p[q] = (k ? wb_f[q] : wb_g[q]) + RB;
+            hi_p[q] = (k ? wb_g[q] : wb_f[q]) + RB;
           }
-          const int64_t lo_c = S.cur[lo_s], hi_c = S.cur[hi_s];
-          const RecT* const recs = reinterpret_cast<const RecT*>(smem);
-          const RecT* lo_p = recs + (S.lbase[lo_s] / SF + static_cast<uint32_t>(lo_k - lo_c));
-          const RecT* hi_p = recs + (S.lbase[hi_s] / SF + static_cast<uint32_t>(hi_k - hi_c));
-          // One merge step: accumulate the segment ending at lo.t, then
-          // replace lo by the next record of its PCF and restore the order.
-          // Two steps per trip let registers alternate roles.
-          auto step = [&]() {
-            acc.add(len, pair_post<OP, TV>(x, p));
-            const RecT r = lds_rec(lo_p);
-            const RecT* np = lo_p + 1;
-            const bool k = r.t <= hi_t;
-            x = pair_pre<OP, TV>(r.v, hi_v);
-            const TT nlt = tmin(r.t, hi_t);
-            len = static_cast<TV>(nlt - lo_t);
-            lo_t = nlt;
-            hi_t = tmax(r.t, hi_t);
-            const RecT* nlp = k ? np : hi_p;
-            hi_p = k ? hi_p : np;
-            lo_p = nlp;
-            hi_v = k ? hi_v : r.v;
-          };
-          int n = 0;
-          for (; n + 2 <= n_uniform; n += 2)
+          else
           {
-            step();
-            step();
+            lo_p[q] = lo_g[q] ? wb_g[q] : wb_f[q];
+            hi_p[q] = lo_g[q] ? wb_f[q] : wb_g[q];
           }
-          if (n < n_uniform)
-            step();
-          while (lo_t < Tb)
+        }
+        // One merge step of pair q on the record r that lo_p[q] pointed at:
+        // accumulate the segment ending at lo.t, then replace lo by r and
+        // restore the order.
+        auto step = [&](int q, const RecT& r) {
+          acc[q].add(len[q], pair_post<OP, TV>(x[q], p));
+          const uint32_t np = lo_p[q] + RB;
+          const bool k = r.t <= hi_t[q];
+          x[q] = pair_pre<OP, TV>(r.v, hi_v[q]);
+          const TT nlt = tmin(r.t, hi_t[q]);
+          len[q] = static_cast<TV>(nlt - lo_t[q]);
+          lo_t[q] = nlt;
+          hi_t[q] = tmax(r.t, hi_t[q]);
+          const uint32_t nlp = k ? np : hi_p[q];
+          hi_p[q] = k ? hi_p[q] : np;
+          lo_p[q] = nlp;
+          hi_v[q] = k ? hi_v[q] : r.v;
+        };
+        // all pairs of the lane, loads first (NCH independent chains)
+        auto step_all = [&]() {
+          RecT r[NCH];
+#pragma unroll
+          for (int q = 0; q < NCH; ++q)
+            r[q] = lds_rec(&S.rec(lo_p[q]));
+#pragma unroll
+          for (int q = 0; q < NCH; ++q)
+            step(q, r[q]);
+        };
+        auto step_one = [&](int q) { step(q, lds_rec(&S.rec(lo_p[q]))); };
+
+        // (1) every pair of every lane: scalar-counted, two steps per trip so
+        // that registers alternate roles instead of being copied
+        const int n_all = __builtin_amdgcn_readfirstlane(static_cast<int>(wave_min(n_min)));
+        int done = 0;
+        for (; done + 2 <= n_all; done += 2)
+        {
+          step_all();
+          step_all();
+        }
+        if (done < n_all)
+        {
+          step_all();
+          ++done;
+        }
+#pragma unroll
+        for (int q = 0; q < NCH; ++q)
+        {
+          // (2) pair q alone up to its own wavefront minimum, then (3) under
+          // the time compare until its next breakpoint reaches Tb
+          if constexpr (NCH > 1)
           {
-            step();
-            if (!(lo_t < Tb))
+            const int nq = __builtin_amdgcn_readfirstlane(static_cast<int>(wave_min(n_steps[q])));
+            int d = done;
+            for (; d + 2 <= nq; d += 2)
+            {
+              step_one(q);
+              step_one(q);
+            }
+            if (d < nq)
+              step_one(q);
+          }
+          while (lo_t[q] < Tb)
+          {
+            step_one(q);
+            if (!(lo_t[q] < Tb))
               break;
-            step();
+            step_one(q);
           }
-          // back to (slot, record index). A next-record pointer lies within
-          // local 0 .. cap of its slot's window (slot_recs > cap), and the
-          // row slot precedes the column slot, so one compare with the
-          // column window's start tells the owners apart.
-          const uint32_t lo_i = static_cast<uint32_t>(lo_p - recs);
-          const uint32_t hi_i = static_cast<uint32_t>(hi_p - recs);
-          const bool lo_in_g = lo_i >= S.lbase[slot_g] / SF;
-          lo_s = lo_in_g ? slot_g : slot_f;
-          hi_s = lo_in_g ? slot_f : slot_g;
-          lo_k = S.cur[lo_s] + (lo_i - S.lbase[lo_s] / SF);
-          hi_k = S.cur[hi_s] + (hi_i - S.lbase[hi_s] / SF);
+        }
+#pragma unroll
+        for (int q = 0; q < NCH; ++q)
+        {
+          // row windows precede column windows
+          lo_g[q] = lo_p[q] >= wb_g[q];
           if (fold)
-            acc.fold();
+            acc[q].fold();
         }
         __syncthreads();
         if (!(Tb < L::inf()))
           break;
       }
 
-      if (valid)
+      // final rectangle [t, MAX) from the last union breakpoint t: the
+      // later of the two PCFs' last record starts
+      auto last_start = [&](bool row, int64_t g) {
+        const int64_t* off = row ? P.r_off : P.c_off;
+        const RecT* rec = static_cast<const RecT*>(row ? P.r_rec : P.c_rec);
+        const int64_t b = off[g], m = off[g + 1] - b;
+        return m >= 2 ? rec[b + m - 2].t : TT(0);
+      };
+#pragma unroll
+      for (int q = 0; q < NCH; ++q)
       {
-        // final rectangle [t, MAX) from the last union breakpoint t: the
-        // later of the two PCFs' last record starts
-        auto last_start = [&](int s, const void* rec) {
-          const int64_t m = S.m[s];
-          return m >= 2 ? static_cast<const RecT*>(rec)[S.base[s] + m - 2].t : TT(0);
-        };
-        const TT t = tmax(last_start(slot_f, P.r_rec), last_start(slot_g, P.c_rec));
+        if (!valid[q])
+          continue;
+        const int c = lc + q * half;
+        const int64_t gr = pr.row_base + tl.r0 + lr;
+        const int64_t gc = pr.col_base + tl.c0 + c;
+        const TT t = tmax(last_start(true, gr), last_start(false, gc));
         const TT tail = L::max() - t;
-        const double raw = acc.finish(tail, pair_post<OP, TV>(x, p));
+        const double raw = acc[q].finish(tail, pair_post<OP, TV>(x[q], p));
         const double res = finalize<OP>(raw, P.p);
-        const int64_t i = P.r_perm[pr.row_base + tl.r0 + lr];
-        const int64_t j = P.c_perm[pr.col_base + tl.c0 + lc];
+        const int64_t i = P.r_perm[gr];
+        const int64_t j = P.c_perm[gc];
         int64_t idx;
         if (P.out_kind == SBAMD_OUT_DISTMAT)
           idx = distmat_index(i, j);
@@ -212,34 +285,36 @@ namespace sbamd
     }
   }
 
-  // LDS of the forward kernel: the staging layout plus R row-sum partials.
+  // LDS of the forward kernel (staging layout, tables, R row-sum partials).
   size_t fwd_lds_bytes(int tt_f64, int tv_f64, int R, int C, int cap)
   {
     (void)tv_f64;
-    return static_cast<size_t>(lds_layout(tt_f64 ? 8 : 4, R, C, cap, 0).total) + 8 * static_cast<size_t>(R);
+    return static_cast<size_t>(lds_layout(tt_f64 ? 8 : 4, R, C, cap).total);
   }
 
-  template <typename TT, typename TV, bool KAHAN>
+  template <typename TT, typename TV, bool KAHAN, int NCH>
   static void launch_typed(const FwdParams& prm, int op, hipStream_t stream)
   {
     const size_t lds = fwd_lds_bytes(sizeof(TT) == 8, 0, prm.R, prm.C, prm.cap);
     if (prm.ntiles <= 0)
       return;
-    const unsigned threads = tile_block_threads(prm.R, prm.C);
+    if (!tile_shape_ok(prm.R, prm.C, prm.cap, NCH))
+      fail(SBAMD_ERR_INVALID, "tile shape not supported by the tile kernel");
+    const unsigned threads = static_cast<unsigned>(tile_block_threads(prm.R, prm.C, NCH));
     void (*kern)(FwdParams) = nullptr;
     switch (op)
     {
     case OPK_L1:
-      kern = fwd_tile_kernel<TT, TV, OPK_L1, KAHAN>;
+      kern = fwd_tile_kernel<TT, TV, OPK_L1, KAHAN, NCH>;
       break;
     case OPK_L2:
-      kern = fwd_tile_kernel<TT, TV, OPK_L2, KAHAN>;
+      kern = fwd_tile_kernel<TT, TV, OPK_L2, KAHAN, NCH>;
       break;
     case OPK_LP:
-      kern = fwd_tile_kernel<TT, TV, OPK_LP, KAHAN>;
+      kern = fwd_tile_kernel<TT, TV, OPK_LP, KAHAN, NCH>;
       break;
     default:
-      kern = fwd_tile_kernel<TT, TV, OPK_DOT, KAHAN>;
+      kern = fwd_tile_kernel<TT, TV, OPK_DOT, KAHAN, NCH>;
       break;
     }
     int64_t grid = prm.ntiles;
@@ -259,15 +334,24 @@ namespace sbamd
     SBAMD_HIP(hipGetLastError());
   }
 
+  template <typename TT, typename TV, bool KAHAN>
+  static void launch_ppl(const FwdParams& prm, int op, hipStream_t stream)
+  {
+    if (prm.ppl == 1)
+      launch_typed<TT, TV, KAHAN, 1>(prm, op, stream);
+    else
+      launch_typed<TT, TV, KAHAN, 2>(prm, op, stream);
+  }
+
   void launch_forward(const FwdParams& prm, int tt_f64, int tv_f64, int op, int kahan,
                       hipStream_t stream)
   {
     if (tt_f64 && tv_f64 && !kahan)
-      launch_typed<double, double, false>(prm, op, stream);
+      launch_ppl<double, double, false>(prm, op, stream);
     else if (tt_f64 && !tv_f64 && kahan)
-      launch_typed<double, float, true>(prm, op, stream);
+      launch_ppl<double, float, true>(prm, op, stream);
     else if (!tt_f64 && !tv_f64 && kahan)
-      launch_typed<float, float, true>(prm, op, stream);
+      launch_ppl<float, float, true>(prm, op, stream);
     else
       fail(SBAMD_ERR_INVALID, "unsupported record layout / precision combination");
   }

@@ -43,6 +43,23 @@ namespace sbamd
   };
   static_assert(sizeof(Tile) == 16, "Tile must stay 16 bytes");
 
+  // Tile kernel geometry. With `ppl` pairs per lane, lane (r, c) of a tile
+  // walks pairs (r, c + q * ceil(C / ppl)), q < ppl; lanes past the tile's
+  // pairs walk a stand-in pair and store nothing. Workgroups are whole
+  // wavefronts of at most 512 lanes; one wavefront reduces the R + C slots'
+  // phase-end proposals, so R + C <= 64.
+  __host__ __device__ inline int tile_lane_cols(int C, int ppl) { return (C + ppl - 1) / ppl; }
+  __host__ __device__ inline int tile_block_threads(int R, int C, int ppl)
+  {
+    const int lanes = R * tile_lane_cols(C, ppl);
+    return lanes <= 64 ? 64 : (lanes + 63) / 64 * 64;
+  }
+  inline bool tile_shape_ok(int R, int C, int cap, int ppl)
+  {
+    return (ppl == 1 || ppl == 2) && R >= 1 && C >= 1 && R + C <= 64 &&
+           tile_block_threads(R, C, ppl) <= 512 && cap >= 2 && cap <= 4096;
+  }
+
   constexpr uint16_t TILE_DIAG = 1;
   constexpr uint16_t TILE_FOLD = 2;
 
@@ -80,6 +97,7 @@ namespace sbamd
     int32_t out_kind;
     int32_t out_f64;
     int32_t R, C, cap, stride;
+    int32_t ppl; // pairs per lane (1 or 2)
     int32_t symmetric; // rows and cols are the same set (pdist / kernel)
     int32_t include_diag;
     uint32_t* work_counter; // non-null: persistent workgroups take tiles from this counter
@@ -243,7 +261,10 @@ namespace sbamd
     int32_t out_kind = SBAMD_OUT_DISTMAT;
     int32_t precision = SBAMD_PREC_FP64;
     int32_t tt_f64 = 1, tv_f64 = 1;
-    int32_t R = 16, C = 16, cap = 48; // measured best on MI355X for m ~ 50..500
+    // default tile: measured best on MI355X for m ~ 50..500 (the autotuner
+    // may pick another)
+    int32_t R = 16, C = 16, cap = 48;
+    int32_t ppl = 1; // pairs per lane of the tile kernel
     int64_t fold_threshold = 8192;
     uint64_t pairs = 0, segments = 0, out_count = 0;
     std::vector<Problem> h_probs;

@@ -226,102 +226,92 @@ namespace sbamd
       return *p;
   }
 
-  // Byte layout of the dynamic LDS block of a tile kernel:
-  //   slot s (rows 0..R-1, then columns), record j at
-  //     rows_off + (s * slot_recs(cap) + j) * REC
-  //   Every slot is contiguous with the same record stride, so advancing a
-  //   walk pointer is one add whichever side it belongs to; slot_recs is 1
-  //   modulo 16 so 16-byte records of different slots at the same position,
-  //   and of one slot at nearby positions, fall in different bank groups.
-  //   Then per-slot tables, the phase end time, and optional per-record extra
-  //   space (backward gradient slots) mirroring the record geometry.
-  constexpr int ROW_PAD = 4;
-
+  // ---------------------------------------------------------------------
+  // LDS of a tile kernel.
+  //
+  // Slot s (rows 0..R-1, then columns R..R+C-1) has a window of sr =
+  // slot_recs(cap) records; record j of slot s sits at
+  //   win_off + (s * sr + j) * RB
+  // with RB the record size (16 bytes for FP64 times, 8 for FP32 times).
+  // Every window has the same stride, so a walk pointer advances by RB
+  // whichever side it belongs to, and a pointer is known to lie in a
+  // column window when it is at or past the first column window (rows
+  // precede columns). sr is 1 modulo 16 so that 16-byte records of
+  // different slots at the same position fall in different bank groups.
+  // Then per-slot tables and the phase end time.
+  // ---------------------------------------------------------------------
   __host__ __device__ inline int slot_recs(int cap) { return (cap + 15) / 16 * 16 + 1; }
-
-  struct LdsLayout
-  {
-    int rec_bytes;
-    int rows_off, cols_off, rec_end;
-    int base_off, m_off, cur_off, next_off, tstart_off, lbase_off, lstride_off, tb_off, misc_off;
-    int extra_off, total;
-  };
 
   __host__ __device__ inline int align_up(int x, int a) { return (x + a - 1) / a * a; }
 
-  // extra_per_rec: bytes per staged record of the optional extra region
-  // (a multiple of rec_bytes; placed at extra_off + rec_offset * extra_per_rec / rec_bytes).
-  __host__ __device__ inline LdsLayout lds_layout(int tt_bytes, int R, int C, int cap, int extra_per_rec)
+  struct LdsLayout
+  {
+    int rb;         // record bytes
+    int sr;         // records per slot window
+    int win_off;    // slot 0's window
+    int src_off;    // u64 per slot: global byte address of the next window's first record
+    int tstart_off; // time type per slot: start time of the next window's first record
+    int rem_off;    // i32 per slot: records from the next window's first to the PCF's end
+    int cnt_off;    // i32 per slot: records consumed in the current phase
+    int tb_off;     // phase end time (bits)
+    int misc_off;   // next tile of a persistent workgroup
+    int rows_off;   // R FP64 row-sum partials
+    int total;
+  };
+
+  __host__ __device__ inline LdsLayout lds_layout(int tt_bytes, int R, int C, int cap)
   {
     LdsLayout L;
     const int NS = R + C;
-    L.rec_bytes = tt_bytes == 8 ? 16 : 8;
+    L.rb = tt_bytes == 8 ? 16 : 8;
+    L.sr = slot_recs(cap);
     int o = 0;
-    L.rows_off = o;
-    o += R * slot_recs(cap) * L.rec_bytes;
-    L.cols_off = o;
-    o += C * slot_recs(cap) * L.rec_bytes;
-    L.rec_end = o;
-    L.base_off = o;
-    o += 8 * NS;
-    L.m_off = o;
-    o += 8 * NS;
-    L.cur_off = o;
-    o += 8 * NS;
-    L.next_off = o;
+    L.win_off = o;
+    o += NS * L.sr * L.rb;
+    L.src_off = o;
     o += 8 * NS;
     L.tstart_off = o;
     o += 8 * NS;
-    L.lbase_off = o;
+    L.rem_off = o;
     o += 4 * NS;
-    L.lstride_off = o;
+    L.cnt_off = o;
     o += 4 * NS;
     o = align_up(o, 8);
     L.tb_off = o;
-    o += 16;
-    L.misc_off = o; // next tile index of a persistent workgroup
-    o += 16;
-    o = align_up(o, 16);
-    L.extra_off = o;
-    o += (L.rec_end / L.rec_bytes) * extra_per_rec;
+    o += 8;
+    L.misc_off = o;
+    o += 8;
+    L.rows_off = o;
+    o += 8 * R;
     L.total = align_up(o, 16);
     return L;
   }
 
-  // Workgroup size of a tile kernel: one lane per pair, whole wavefronts.
-  inline unsigned tile_block_threads(int R, int C)
-  {
-    const int lanes = R * C;
-    return static_cast<unsigned>(lanes <= 64 ? 64 : (lanes + 63) / 64 * 64);
-  }
-
-  // ---------------------------------------------------------------------
-  // Phase staging shared by the forward and backward tile kernels.
-  // ---------------------------------------------------------------------
   template <typename TT, typename TV>
   struct TileLds
   {
     unsigned char* smem;
     LdsLayout lay;
-    int64_t* base;   // global record offset of each slot's PCF
-    int64_t* m;      // records of each slot's PCF (0 for an unused slot)
-    int64_t* cur;    // first record staged this phase
-    int64_t* next;   // first record starting at or after this phase's end
-    TT* tstart;      // start time of record `next` (phase bookkeeping)
-    uint32_t* lbase; // LDS byte address of record 0 of the slot's phase window
-    uint32_t* lstride;
+    unsigned long long* src;
+    TT* tstart;
+    int32_t* rem;
+    int32_t* cnt;
     typename Lim<TT>::Bits* tb;
 
     __device__ TileLds(unsigned char* s, const LdsLayout& L) : smem(s), lay(L)
     {
-      base = reinterpret_cast<int64_t*>(s + L.base_off);
-      m = reinterpret_cast<int64_t*>(s + L.m_off);
-      cur = reinterpret_cast<int64_t*>(s + L.cur_off);
-      next = reinterpret_cast<int64_t*>(s + L.next_off);
+      src = reinterpret_cast<unsigned long long*>(s + L.src_off);
       tstart = reinterpret_cast<TT*>(s + L.tstart_off);
-      lbase = reinterpret_cast<uint32_t*>(s + L.lbase_off);
-      lstride = reinterpret_cast<uint32_t*>(s + L.lstride_off);
+      rem = reinterpret_cast<int32_t*>(s + L.rem_off);
+      cnt = reinterpret_cast<int32_t*>(s + L.cnt_off);
       tb = reinterpret_cast<typename Lim<TT>::Bits*>(s + L.tb_off);
+    }
+
+    // LDS byte address of record 0 of slot s's window
+    __device__ __forceinline__ uint32_t win(int s) const
+    {
+      return static_cast<uint32_t>(lay.win_off) +
+             static_cast<uint32_t>(s) * static_cast<uint32_t>(lay.sr * lay.rb);
     }
 
     __device__ __forceinline__ Rec<TT, TV>& rec(uint32_t addr) const
@@ -329,45 +319,6 @@ namespace sbamd
       return *reinterpret_cast<Rec<TT, TV>*>(smem + addr);
     }
   };
-
-  // Tile start: slot s < R is row PCF r0 + s, slot R + c is column c0 + c.
-  template <typename TT, typename TV>
-  __device__ void tile_begin(const TileLds<TT, TV>& S, const FwdParams& P, const Tile& tl,
-                             const Problem& pr, int tid)
-  {
-    using L = Lim<TT>;
-    const int NS = P.R + P.C;
-    if (tid < NS)
-    {
-      const bool isrow = tid < P.R;
-      const int k = isrow ? tid : tid - P.R;
-      const int lim = isrow ? tl.nr : tl.nc;
-      const int64_t* off = isrow ? P.r_off : P.c_off;
-      if (k < lim)
-      {
-        const int64_t g = isrow ? pr.row_base + tl.r0 + k : pr.col_base + tl.c0 + k;
-        const int64_t b = off[g];
-        S.base[tid] = b;
-        S.m[tid] = off[g + 1] - b;
-      }
-      else
-      {
-        S.base[tid] = 0;
-        S.m[tid] = 0;
-      }
-      S.next[tid] = 0;
-      S.tstart[tid] = TT(0);
-      const uint32_t rb = static_cast<uint32_t>(S.lay.rec_bytes);
-      S.lbase[tid] = static_cast<uint32_t>(S.lay.rows_off) +
-                     static_cast<uint32_t>(tid * slot_recs(P.cap)) * rb;
-      S.lstride[tid] = rb;
-    }
-    if (tid == 0)
-    {
-      S.tb[0] = L::bits(L::inf());
-      S.tb[1] = L::bits(L::inf());
-    }
-  }
 
   // One 16-byte global -> LDS load (global_load_lds_dwordx4). lds_base is the
   // wavefront-uniform LDS address of lane 0; lane i's record lands at
@@ -381,21 +332,6 @@ namespace sbamd
     (void)src;
     (void)lds_base;
 #endif
-  }
-
-  // A value every lane of the wavefront holds (read from one LDS address),
-  // moved to a scalar register so that the address arithmetic on it is
-  // scalar.
-  __device__ __forceinline__ int64_t wave_uniform(int64_t x)
-  {
-    const uint64_t u = static_cast<uint64_t>(x);
-    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(u)));
-    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(u >> 32)));
-    return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
-  }
-  __device__ __forceinline__ uint32_t wave_uniform(uint32_t x)
-  {
-    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
   }
 
   // Minimum over the 64 lanes of a wavefront (all lanes active), as a
@@ -453,116 +389,122 @@ namespace sbamd
     return r;
   }
 
-  // First half of a phase (before the barrier): each slot proposes the phase
-  // end (start time of its (cap+1)-th unconsumed record), and the workgroup
-  // copies up to cap records of every slot into LDS, one wavefront per slot
-  // (a slot's window is contiguous in global memory and in LDS, so each lane
-  // moves whole 16- or 8-byte records). With LEAD = 1 the window also holds
-  // the record before the cursor (the record a walk is inside when the phase
-  // starts) at local index 0, and record cur + j at local j + 1.
-  template <typename TT, typename TV, int LEAD = 0>
-  __device__ void phase_stage(const TileLds<TT, TV>& S, const FwdParams& P, int phase, int tid,
-                              int nthreads)
-  {
-    using L = Lim<TT>;
-    using RecT = Rec<TT, TV>;
-    const int NS = P.R + P.C;
-    const int cap = P.cap;
-    const RecT* __restrict__ g_r = static_cast<const RecT*>(P.r_rec);
-    const RecT* __restrict__ g_c = static_cast<const RecT*>(P.c_rec);
-    if ((tid & ~63) < NS)
-    {
-      // the proposals of a wavefront's slot threads are reduced in registers
-      // (times are >= 0, so their bit patterns order as the times do) and
-      // entered with one LDS atomic per wavefront
-      typename L::Bits tbs = L::bits(L::inf());
-      if (tid < NS)
-      {
-        const int64_t c = S.next[tid];
-        S.cur[tid] = c;
-        if (S.m[tid] - c > cap)
-          tbs = L::bits((tid < P.R ? g_r : g_c)[S.base[tid] + c + cap - 1].t);
-      }
-      tbs = wave_min(tbs);
-      if ((tid & 63) == 0)
-        atomicMin(&S.tb[phase & 1], tbs);
-    }
-    const int lane = tid & 63;
-    const int nwaves = nthreads >> 6;
-    if constexpr (sizeof(RecT) == 16)
-    {
-      // Direct global -> LDS loads (global_load_lds_dwordx4): lane i of a
-      // 64-record chunk lands at chunk base + 16 i, which is exactly the
-      // window layout, so records bypass registers and every slot's loads
-      // are in flight together; one vmcnt wait below covers them. The slot
-      // tables are read per lane (one broadcast LDS read each) and the
-      // window address comes from the slot index, so a slot costs a few
-      // vector operations and no readfirstlane round trips.
-      const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-      const uint32_t slot_bytes = static_cast<uint32_t>(slot_recs(cap)) * 16u;
-      for (int s = w; s < NS; s += nwaves)
-      {
-        const int64_t m = S.m[s];
-        const int64_t first = S.next[s] - LEAD; // window record 0
-        // window records j in [jlo, jhi): record first + j exists
-        const int jlo = first < 0 ? 1 : 0;
-        const int jhi = static_cast<int>(m - first < cap + LEAD ? m - first : cap + LEAD);
-        const RecT* src = (s < P.R ? g_r : g_c) + (S.base[s] + first);
-        const uint32_t dst = static_cast<uint32_t>(S.lay.rows_off) + static_cast<uint32_t>(s) * slot_bytes;
-        for (int c0 = 0; c0 < cap + LEAD; c0 += 64)
-        {
-          const int j = c0 + lane;
-          if (j >= jlo && j < jhi)
-            lds_dma16(src + j, S.smem + dst + 16u * static_cast<uint32_t>(c0));
-        }
-      }
-    }
-    else
-    {
-      for (int s = tid >> 6; s < NS; s += nwaves)
-      {
-        const int64_t m = wave_uniform(S.m[s]);
-        const int64_t first = wave_uniform(S.next[s]) - LEAD; // window record 0
-        const int jlo = first < 0 ? 1 : 0;
-        const int jhi = static_cast<int>(m - first < cap + LEAD ? m - first : cap + LEAD);
-        const RecT* src = (s < P.R ? g_r : g_c) + wave_uniform(S.base[s]);
-        const uint32_t dst = wave_uniform(S.lbase[s]);
-        for (int j = lane; j < jhi; j += 64)
-          if (j >= jlo)
-            S.rec(dst + static_cast<uint32_t>(j) * static_cast<uint32_t>(sizeof(RecT))) = src[first + j];
-      }
-    }
-    if constexpr (sizeof(RecT) == 16)
-      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    if (tid == 0)
-      S.tb[(phase + 1) & 1] = L::bits(L::inf());
-  }
-
-  // After the barrier: the slot threads find the first staged record that
-  // starts at or after Tb (binary search) -- the next phase's cursor.
-  template <typename TT, typename TV, int LEAD = 0>
-  __device__ void phase_advance_cursor(const TileLds<TT, TV>& S, const FwdParams& P, TT Tb, int tid)
+  // Tile start: slot s < R is row PCF r0 + s, slot R + c is column c0 + c;
+  // slots past the tile's extent hold no records.
+  template <typename TT, typename TV>
+  __device__ void tile_begin(const TileLds<TT, TV>& S, const FwdParams& P, const Tile& tl,
+                             const Problem& pr, int tid)
   {
     const int NS = P.R + P.C;
     if (tid < NS)
     {
-      const int64_t c = S.cur[tid];
-      const int64_t rem = S.m[tid] - c;
-      const int lim = rem < P.cap ? static_cast<int>(rem) : P.cap;
-      const uint32_t lb = S.lbase[tid], ls = S.lstride[tid];
-      int lo = 0, hi = lim;
+      const bool isrow = tid < P.R;
+      const int k = isrow ? tid : tid - P.R;
+      const int lim = isrow ? tl.nr : tl.nc;
+      const int64_t* off = isrow ? P.r_off : P.c_off;
+      const unsigned char* recs = static_cast<const unsigned char*>(isrow ? P.r_rec : P.c_rec);
+      int64_t b = 0, m = 0;
+      if (k < lim)
+      {
+        const int64_t g = isrow ? pr.row_base + tl.r0 + k : pr.col_base + tl.c0 + k;
+        b = off[g];
+        m = off[g + 1] - b;
+      }
+      S.src[tid] = reinterpret_cast<unsigned long long>(recs + b * S.lay.rb);
+      S.rem[tid] = static_cast<int32_t>(m);
+      S.tstart[tid] = TT(0);
+      S.cnt[tid] = 0;
+    }
+  }
+
+  // First part of a phase (before the barrier): every slot's next window
+  // (up to cap records from its cursor) is copied into LDS, one wavefront
+  // per slot, each lane moving whole records; with 16-byte records the
+  // copies go global -> LDS directly (global_load_lds_dwordx4), so one
+  // vmcnt wait covers them all. Wavefront 0 also finds the phase end: each
+  // slot with more than cap records left proposes the end time of the last
+  // record it stages, and the phase ends at the earliest proposal (+inf
+  // when every slot stages all it has left).
+  template <typename TT, typename TV>
+  __device__ void phase_stage(const TileLds<TT, TV>& S, const FwdParams& P, int tid, int nthreads)
+  {
+    using L = Lim<TT>;
+    using RecT = Rec<TT, TV>;
+    constexpr uint32_t RB = sizeof(RecT);
+    const int NS = P.R + P.C;
+    const int cap = P.cap;
+    const int lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nwaves = nthreads >> 6;
+    typename L::Bits prop = L::bits(L::inf());
+    if (w == 0 && lane < NS && S.rem[lane] > cap)
+    {
+      const RecT* last = reinterpret_cast<const RecT*>(S.src[lane]) + (cap - 1);
+      prop = L::bits(last->t);
+    }
+    for (int s = w; s < NS; s += nwaves)
+    {
+      const int rem = S.rem[s];
+      const int n = rem < cap ? rem : cap;
+      const unsigned char* src = reinterpret_cast<const unsigned char*>(S.src[s]);
+      const uint32_t dst = S.win(s);
+      for (int c0 = 0; c0 < n; c0 += 64)
+      {
+        const int j = c0 + lane;
+        if (j < n)
+        {
+          if constexpr (RB == 16)
+            lds_dma16(src + static_cast<size_t>(j) * RB, S.smem + dst + RB * static_cast<uint32_t>(c0));
+          else
+            S.rec(dst + RB * static_cast<uint32_t>(j)) = reinterpret_cast<const RecT*>(src)[j];
+        }
+      }
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    if (w == 0)
+    {
+      const typename L::Bits tb = wave_min(prop);
+      if (lane == 0)
+        *S.tb = tb;
+    }
+  }
+
+  // Second part (after the barrier): the cursor of every slot -- the number
+  // of staged records that start before the phase end Tb, found by a binary
+  // search over the window (starts increase along it) -- and the slot's next
+  // window: its first record, start time and remaining count. One lane of
+  // wavefront 0 per slot, all slots at once.
+  template <typename TT, typename TV>
+  __device__ void phase_cursors(const TileLds<TT, TV>& S, const FwdParams& P, TT Tb, int tid)
+  {
+    using RecT = Rec<TT, TV>;
+    constexpr uint32_t RB = sizeof(RecT);
+    const int NS = P.R + P.C;
+    if (tid < NS)
+    {
+      const int rem = S.rem[tid];
+      const int n = rem < P.cap ? rem : P.cap;
+      const uint32_t wb = S.win(tid);
+      const TT t0 = S.tstart[tid];
+      // first j in [0, n) whose record starts at or after Tb (n if none);
+      // record j starts where record j - 1 ends, record 0 at t0
+      int lo = 0, hi = n;
       while (lo < hi)
       {
         const int mid = (lo + hi) >> 1;
-        const TT ts = mid == 0 ? S.tstart[tid] : S.rec(lb + static_cast<uint32_t>(mid - 1 + LEAD) * ls).t;
-        if (ts < Tb)
+        const TT st = mid == 0 ? t0 : S.rec(wb + RB * static_cast<uint32_t>(mid - 1)).t;
+        if (st < Tb)
           lo = mid + 1;
         else
           hi = mid;
       }
-      S.next[tid] = c + lo;
+      S.cnt[tid] = lo;
       if (lo > 0)
-        S.tstart[tid] = S.rec(lb + static_cast<uint32_t>(lo - 1 + LEAD) * ls).t;
+      {
+        S.tstart[tid] = S.rec(wb + RB * static_cast<uint32_t>(lo - 1)).t;
+        S.rem[tid] = rem - lo;
+        S.src[tid] += static_cast<unsigned long long>(lo) * RB;
+      }
     }
   }
 }

@@ -391,7 +391,9 @@ class Plan:
         self.out_kind = out_kind
         sh = L.TileShape()
         if shape:
-            sh.rows, sh.cols, sh.cap = shape
+            sh.rows, sh.cols, sh.cap = shape[:3]
+            if len(shape) > 3:
+                sh.lane_pairs = shape[3]
         h = ctypes.c_void_p()
         if batched:
             arr = (ctypes.c_void_p * len(self._sets))(*[s.handle.value for s in self._sets])
@@ -415,6 +417,7 @@ class Plan:
         self.tiles = st.tiles
         self.input_bytes = st.input_bytes
         self.shape = (st.rows, st.cols, st.cap)
+        self.lane_pairs = st.lane_pairs
         self.problems = st.problems
         self.long_pairs = bool(st.long_pairs)
 
@@ -435,8 +438,8 @@ class Plan:
     def set_persistent(self, on: bool):
         L.check(L.lib().sbamd_plan_set_persistent(self.handle, 1 if on else 0))
 
-    def set_shape(self, rows, cols, cap):
-        sh = L.TileShape(rows, cols, cap, 0)
+    def set_shape(self, rows, cols, cap, lane_pairs=0):
+        sh = L.TileShape(rows, cols, cap, lane_pairs)
         L.check(L.lib().sbamd_plan_set_shape(self.handle, ctypes.byref(sh)))
         self.refresh_stats()
 

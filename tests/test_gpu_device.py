@@ -28,12 +28,14 @@ def _run(plan, f64):
     return out.cpu().numpy()
 
 
-@pytest.mark.parametrize("shape", [(16, 16, 24), (16, 16, 48), (8, 8, 32), (16, 16, 2), (4, 4, 7)])
+@pytest.mark.parametrize("shape", [(16, 16, 24, 1), (16, 16, 48, 1), (8, 8, 32, 1), (16, 16, 2, 1),
+                                   (4, 4, 7, 1), (32, 32, 48, 2), (32, 32, 64, 2), (16, 16, 24, 2),
+                                   (8, 8, 32, 2), (16, 16, 2, 2), (4, 4, 7, 2), (5, 5, 9, 2)])
 def test_tile_shapes_all_bit_exact(gpu, shape):
     off, t, v = workloads.c2(90, seed=3)
     exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_PDIST, off, t, v)
     ds, pl = _plan(off, t, v, shape=shape)
-    assert pl.shape == shape
+    assert pl.shape == shape[:3] and pl.lane_pairs == shape[3]
     np.testing.assert_array_equal(_run(pl, True), exp)
 
 
