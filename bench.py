@@ -362,7 +362,8 @@ def main(argv=None):
     fwdbwd = None
     if not args.no_fwdbwd:
         fwdbwd = _fwd_bwd_c4(stream)
-        _progress(f"fwd+bwd: ratio {fwdbwd['ratio']:.2f}")
+        _progress(f"fwd+bwd: pdist ratio {fwdbwd['l2_pdist']['ratio']:.2f}, "
+                  f"kernel ratio {fwdbwd['l2_kernel']['ratio']:.2f}")
 
     line = {
         "metric": METRIC,
@@ -403,48 +404,81 @@ def main(argv=None):
 
 
 def _fwd_bwd_c4(stream):
-    """L2 pdist forward + backward of sum(W * D), N = 4096, m ~ U{20..200},
-    FP32C forward and FP64-accumulated backward."""
+    """Forward + backward at N = 4096, m ~ U{20..200}, through the
+    torch.autograd functions (FP32C forward, FP64-accumulated backward): the
+    L2 pdist with loss sum(W * D) and the L2 kernel with loss sum(W * K).
+    Times of a forward (the Function's forward and the loss) and of a forward
+    and backward (loss.backward()), events on the current stream around the
+    whole Python calls; the plan-level kernel times beside them."""
     import torch
 
-    from stablebear_amd import runtime, workloads
+    from stablebear_amd import autograd, workloads
 
     off, t, v, w = workloads.c4()
-    csr = runtime.csr_from_arrays(off, t, v)
-    ds = runtime.DeviceSet(csr, "fp32c")
-    plan = runtime.Plan(ds, op="lp", p=2.0)
-    out = torch.empty(plan.out_count, dtype=torch.float32, device="cuda")
-    G = torch.from_numpy(2.0 * w).float().cuda()
-    gv = torch.empty(len(v), dtype=torch.float32, device="cuda")
-    gt = torch.empty(len(t), dtype=torch.float32, device="cuda")
-    s = stream.cuda_stream
+    n = len(off) - 1
+    res = {"workload": "N=4096 m~U{20..200}: L2 pdist loss sum(W*D), l2_kernel loss sum(W*K), "
+                       "through torch.autograd (loss.backward())"}
+    off_d = torch.tensor(off, dtype=torch.int64, device="cuda")
+    tt = torch.tensor(t, dtype=torch.float64, device="cuda", requires_grad=True)
+    vt = torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True)
+    for name, fn, W in (("l2_pdist", autograd.pdist_l2, w),
+                        ("l2_kernel", autograd.l2_kernel,
+                         np.random.default_rng(11).uniform(0, 1, n * (n + 1) // 2))):
+        Wt = torch.tensor(W, dtype=torch.float32, device="cuda")
 
-    def fwd():
+        def fwd():
+            return (Wt * fn(vt, tt, off_d, precision="fp32c")).sum()
+
+        def fwd_bwd():
+            vt.grad = None
+            tt.grad = None
+            fwd().backward()
+
+        for _ in range(3):
+            fwd_bwd()
+        torch.cuda.synchronize()
+        reps = 10
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record(stream)
+        for _ in range(reps):
+            with torch.no_grad():
+                fwd()
+        e1.record(stream)
+        for _ in range(reps):
+            fwd_bwd()
+        e2.record(stream)
+        torch.cuda.synchronize()
+        f = e0.elapsed_time(e1) / reps
+        fb = e1.elapsed_time(e2) / reps
+        # the kernels alone, on the cached plan
+        entry = next(e for b in autograd._cache.values() for e in b
+                     if e.plan.out_count == (n * (n - 1) // 2 if name == "l2_pdist" else n * (n + 1) // 2))
+        plan = entry.plan
+        out = torch.empty(plan.out_count, dtype=torch.float32, device="cuda")
+        G = torch.from_numpy(W).float().cuda()
+        gv = torch.empty(len(v), dtype=torch.float32, device="cuda")
+        gt = torch.empty(len(t), dtype=torch.float32, device="cuda")
+        s = stream.cuda_stream
+        k0, k1, k2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         plan.execute(out.data_ptr(), False, stream=s)
-
-    def bwd():
         plan.backward(G.data_ptr(), out.data_ptr(), False, gv.data_ptr(), gt.data_ptr(), False,
                       stream=s)
-
-    for _ in range(2):
-        fwd()
-        bwd()
-    torch.cuda.synchronize()
-    reps = 5
-    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-    e0.record(stream)
-    for _ in range(reps):
-        fwd()
-    e1.record(stream)
-    for _ in range(reps):
-        fwd()
-        bwd()
-    e2.record(stream)
-    torch.cuda.synchronize()
-    f = e0.elapsed_time(e1) / reps
-    fb = e1.elapsed_time(e2) / reps
-    return {"workload": "L2 pdist N=4096 m~U{20..200}, loss sum(W*D)", "fwd_ms": f,
-            "fwd_bwd_ms": fb, "ratio": fb / f}
+        torch.cuda.synchronize()
+        k0.record(stream)
+        for _ in range(reps):
+            plan.execute(out.data_ptr(), False, stream=s)
+        k1.record(stream)
+        for _ in range(reps):
+            plan.backward(G.data_ptr(), out.data_ptr(), False, gv.data_ptr(), gt.data_ptr(), False,
+                          stream=s)
+        k2.record(stream)
+        torch.cuda.synchronize()
+        kf = k0.elapsed_time(k1) / reps
+        kb = k1.elapsed_time(k2) / reps
+        res[name] = {"fwd_ms": f, "fwd_bwd_ms": fb, "ratio": fb / f,
+                     "kernel_fwd_ms": kf, "kernel_bwd_ms": kb, "kernel_ratio": (kf + kb) / kf}
+    res["ratio"] = res["l2_pdist"]["ratio"]
+    return res
 
 
 def _batched_leg(args, stream):

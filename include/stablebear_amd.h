@@ -178,7 +178,10 @@ int sbamd_set_lp_norm(sbamd_set* set, double p, void* out_device, int32_t out_dt
 int sbamd_plan_create(sbamd_ctx* ctx, const sbamd_set* rows, const sbamd_set* cols, int32_t op,
                       double p, int32_t out_kind, const sbamd_tile_shape* shape, sbamd_plan** out);
 /* Batched pdist / l2_kernel over k independent sets in one launch; outputs
- * are concatenated (problem b at sbamd_plan_out_offset(plan, b)). */
+ * are concatenated (problem b at sbamd_plan_out_offset(plan, b)). The plan
+ * keeps its own copy of the member sets' records; members whose values were
+ * updated (sbamd_set_update_values) are copied again, on the launch stream,
+ * by every execute and graph launch. The member sets must outlive the plan. */
 int sbamd_plan_create_batched(sbamd_ctx* ctx, int32_t k, const sbamd_set* const* sets, int32_t op,
                               double p, int32_t out_kind, const sbamd_tile_shape* shape,
                               sbamd_plan** out);
@@ -238,7 +241,12 @@ int sbamd_plan_backward(sbamd_plan* plan, const void* grad_out_device, const voi
                         int32_t out_dtype, void* grad_v_device, void* grad_t_device,
                         int32_t grad_dtype, void* stream);
 
-/* ---- hipGraph capture of a plan's forward launch ---- */
+/* ---- hipGraph capture of a plan's forward launch ----
+ * The graph binds out_device, the plan's tile list and launch parameters and
+ * the sets' device arrays. sbamd_graph_launch refuses (SBAMD_ERR_INVALID) a
+ * graph whose plan was since reshaped (sbamd_plan_set_shape), given a new
+ * fold threshold, long-pair mode or persistence, autotuned or destroyed;
+ * capture a new graph then. Value updates of the sets need no new capture. */
 int sbamd_graph_create(sbamd_plan* plan, void* out_device, int32_t out_dtype, sbamd_graph** out);
 int sbamd_graph_launch(sbamd_graph* graph, void* stream);
 int sbamd_graph_destroy(sbamd_graph* graph);

@@ -16,6 +16,7 @@
 #include <new>
 #include <thread>
 #include <type_traits>
+#include <unordered_map>
 
 namespace sbamd
 {
@@ -102,6 +103,30 @@ namespace sbamd
     // (legacy default) stream, as everywhere in HIP. The context's own stream
     // is available through sbamd_ctx_stream().
     hipStream_t pick_stream(const Ctx*, void* stream) { return static_cast<hipStream_t>(stream); }
+
+    // Live plans and their launch-state generations, so that a captured
+    // graph can tell whether its plan still holds the buffers and
+    // parameters it was captured with.
+    std::mutex g_plans_mu;
+    std::unordered_map<const Plan*, uint64_t>& live_plans()
+    {
+      static std::unordered_map<const Plan*, uint64_t> m;
+      return m;
+    }
+    std::atomic<uint64_t> g_plan_gen{0};
+
+    void forget_plan(const Plan* pl)
+    {
+      std::lock_guard<std::mutex> lk(g_plans_mu);
+      live_plans().erase(pl);
+    }
+
+    bool plan_current(const Plan* pl, uint64_t gen)
+    {
+      std::lock_guard<std::mutex> lk(g_plans_mu);
+      auto it = live_plans().find(pl);
+      return it != live_plans().end() && it->second == gen;
+    }
 
     template <typename T>
     void upload(DeviceBuffer& buf, const std::vector<T>& v)
@@ -556,6 +581,26 @@ namespace sbamd
         pl.use_long = pl.precision == SBAMD_PREC_FP32C &&
                       pl.pairs * 4 <= static_cast<uint64_t>(pl.ctx->cu_count) * 1536 &&
                       pl.segments >= pl.pairs * 2048;
+      pl.bump();
+    }
+
+    // A batched plan reads its own copy of the member sets' records: copy
+    // again (on `stream`, into the same buffer) the members whose values were
+    // updated since.
+    void refresh_batch(Plan& pl, hipStream_t stream)
+    {
+      const size_t rb = static_cast<size_t>(record_bytes(pl.tt_f64));
+      for (size_t b = 0; b < pl.sets.size(); ++b)
+      {
+        const Set* s = pl.sets[b];
+        if (s->vgen == pl.b_vgen[b])
+          continue;
+        if (s->records > 0)
+          SBAMD_HIP(hipMemcpyAsync(static_cast<char*>(pl.b_rec.ptr) + static_cast<size_t>(pl.b_at[b]) * rb,
+                                   s->rec.ptr, static_cast<size_t>(s->records) * rb,
+                                   hipMemcpyDeviceToDevice, stream));
+        pl.b_vgen[b] = s->vgen;
+      }
     }
 
     void execute_range(const Plan& pl, void* out, int32_t out_dtype, double* row_sum, int64_t t0,
@@ -616,6 +661,13 @@ namespace sbamd
     }
   }
 
+  void Plan::bump()
+  {
+    gen = ++g_plan_gen;
+    std::lock_guard<std::mutex> lk(g_plans_mu);
+    live_plans()[this] = gen;
+  }
+
   void Plan::rebuild_tiles()
   {
     h_tiles.clear();
@@ -656,6 +708,7 @@ namespace sbamd
       h_tiles.swap(sorted);
     }
     finish_plan(*this);
+    bump();
   }
 }
 
@@ -678,6 +731,8 @@ struct sbamd_job
 struct sbamd_graph
 {
   Ctx* ctx = nullptr;
+  sbamd_plan* plan = nullptr; // the plan captured, valid while its generation is `gen`
+  uint64_t gen = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -940,6 +995,7 @@ extern "C"
       launch_update_values(v_device, set->src.as<int64_t>(), set->records, set->rec.ptr,
                            record_bytes(set->tt_f64), set->in_dtype == SBAMD_F64, set->tv_f64,
                            pick_stream(set->ctx, stream));
+      ++set->vgen;
     });
   }
 
@@ -1108,6 +1164,8 @@ extern "C"
       for (int32_t b = 0; b < k; ++b)
       {
         const Set* s = sets[b];
+        pl->b_at.push_back(at);
+        pl->b_vgen.push_back(s->vgen);
         if (s->records == 0)
           continue;
         SBAMD_HIP(hipMemcpy(static_cast<char*>(pl->b_rec.ptr) + at * rb, s->rec.ptr,
@@ -1138,6 +1196,7 @@ extern "C"
     return guarded([&] {
       if (!plan)
         return;
+      forget_plan(plan);
       use_device(plan->ctx);
       delete plan;
     });
@@ -1202,6 +1261,7 @@ extern "C"
     return guarded([&] {
       need(plan, "plan");
       plan->persistent = on != 0;
+      plan->bump();
     });
   }
 
@@ -1239,6 +1299,8 @@ extern "C"
       hipStream_t st = pick_stream(plan->ctx, stream);
       if (row_sum_device)
         launch_fill_zero(row_sum_device, static_cast<size_t>(plan->sets[0]->n) * sizeof(double), st);
+      if (plan->batched)
+        refresh_batch(*plan, st);
       execute_range(*plan, out_device, out_dtype, static_cast<double*>(row_sum_device), 0,
                     work_units(*plan), st);
     });
@@ -1255,6 +1317,8 @@ extern "C"
       use_device(plan->ctx);
       const size_t eb = out_dtype == SBAMD_F64 ? 8 : 4;
       DeviceBuffer dout(static_cast<size_t>(plan->out_count) * eb);
+      if (plan->batched)
+        refresh_batch(*plan, plan->ctx->stream);
       execute_range(*plan, dout.ptr, out_dtype, nullptr, 0, work_units(*plan),
                     plan->ctx->stream);
       SBAMD_HIP(hipStreamSynchronize(plan->ctx->stream));
@@ -1428,6 +1492,10 @@ extern "C"
       use_device(plan->ctx);
       auto g = std::make_unique<sbamd_graph>();
       g->ctx = plan->ctx;
+      g->plan = plan;
+      if (plan->batched)
+        refresh_batch(*plan, plan->ctx->stream);
+      g->gen = plan->gen;
       hipStream_t cs;
       SBAMD_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
       hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
@@ -1460,8 +1528,14 @@ extern "C"
   {
     return guarded([&] {
       need(graph, "graph");
+      if (!plan_current(graph->plan, graph->gen))
+        fail(SBAMD_ERR_INVALID, "graph is stale: its plan was reshaped, retuned or destroyed "
+                                "after capture; capture a new graph");
       use_device(graph->ctx);
-      SBAMD_HIP(hipGraphLaunch(graph->exec, pick_stream(graph->ctx, stream)));
+      hipStream_t st = pick_stream(graph->ctx, stream);
+      if (graph->plan->batched)
+        refresh_batch(*graph->plan, st);
+      SBAMD_HIP(hipGraphLaunch(graph->exec, st));
     });
   }
 
@@ -1491,6 +1565,8 @@ extern "C"
         reps = 1;
       use_device(plan->ctx);
       hipStream_t st = plan->ctx->stream;
+      if (plan->batched)
+        refresh_batch(*plan, st);
       const int R0 = plan->R, C0 = plan->C, cap0 = plan->cap, ppl0 = plan->ppl;
       plan->use_long = false; // tile candidates first
       struct Cand
@@ -1613,6 +1689,7 @@ extern "C"
       (void)hipEventDestroy(e1);
       plan->use_long = chose_long;
       plan->long_mode = chose_long ? 1 : 0;
+      plan->bump();
       if (best)
       {
         best->rows = bc.R;

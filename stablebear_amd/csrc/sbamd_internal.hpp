@@ -245,6 +245,7 @@ namespace sbamd
     std::vector<int64_t> h_roff; // sorted record offsets
     std::vector<int64_t> h_perm;
     bool has_empty = false; // some PCF has no points (its record has no source point)
+    uint64_t vgen = 0;      // bumped by every value update (batched plans copy records)
     // rec: one record per constant piece, {t_end, value} in the set's time and
     // value types (16 bytes with FP64 times, 8 with FP32 times), sorted order
     DeviceBuffer roff, rec, perm, src;
@@ -270,8 +271,12 @@ namespace sbamd
     std::vector<Problem> h_probs;
     std::vector<Tile> h_tiles;
     DeviceBuffer probs, tiles;
-    // batched: concatenated set arrays
+    // batched: concatenated set arrays; member b's records start at record
+    // b_at[b] of b_rec, copied when the member's values were at generation
+    // b_vgen[b]
     DeviceBuffer b_roff, b_rec, b_perm;
+    std::vector<int64_t> b_at;
+    std::vector<uint64_t> b_vgen;
     // persistent launch: resident workgroups pull tiles from this counter
     bool persistent = false;
     DeviceBuffer counter;
@@ -283,6 +288,11 @@ namespace sbamd
     int32_t long_mode = 2;
     bool use_long = false;
     uint64_t input_bytes = 0;
+    // Launch state generation: changes whenever the device buffers or launch
+    // parameters a captured graph holds are replaced (a unique number per
+    // plan state across the process).
+    uint64_t gen = 0;
+    void bump();
     void rebuild_tiles();
   };
 

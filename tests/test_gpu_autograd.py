@@ -98,3 +98,107 @@ def test_c4_full_size_gradient(gpu):
     assert np.max(np.abs(gv - ev)) <= 2e-5 * np.abs(ev).max()
     m = _interior_times(off)
     assert np.max(np.abs(gt[m] - et[m])) <= 2e-5 * np.abs(et).max()
+
+
+def test_c4_full_size_kernel_gradient(gpu):
+    """C4 l2_kernel at full size (N = 4096): FP32C forward against the FP64
+    oracle (normwise: the Gram entries are signed sums) and backward of
+    sum(W * K) against the CPU analytic gradient."""
+    off, t, v, _ = _inputs(4096)
+    n = len(off) - 1
+    W = np.random.default_rng(11).uniform(0, 1, n * (n + 1) // 2)
+    out, gv, gt = _grads("kernel", "fp32c", off, t, v, W)
+    fwd = oracle.pairwise(oracle.OP_DOT, 1.0, oracle.KIND_KERNEL, off, t, v)
+    scale = oracle.pairwise(oracle.OP_DOT, 1.0, oracle.KIND_KERNEL, off, t, np.abs(v))
+    assert np.max(np.abs(out - fwd) / scale) <= 1e-6
+    ev, et = oracle.grad(oracle.OP_DOT, oracle.KIND_KERNEL, off, t, v, W, None)
+    assert np.max(np.abs(gv - ev)) <= 2e-5 * np.abs(ev).max()
+    m = _interior_times(off)
+    assert np.max(np.abs(gt[m] - et[m])) <= 2e-5 * np.abs(et).max()
+
+
+def test_c4_full_size_kernel_finite_differences(gpu):
+    """Central finite differences (h = 1e-6) of sum(W * K) at N = 4096, the
+    loss evaluated by the FP64 GPU forward, against the FP64 backward, on
+    sampled value and interior-time coordinates away from coincident
+    breakpoints."""
+    off, t, v, _ = _inputs(4096)
+    n = len(off) - 1
+    W = np.random.default_rng(12).uniform(0, 1, n * (n + 1) // 2)
+    _, gv, gt = _grads("kernel", "fp64", off, t, v, W)
+    Wd = torch.tensor(W, dtype=torch.float64, device="cuda")
+    off_d = torch.tensor(off, dtype=torch.int64, device="cuda")
+    owner = np.repeat(np.arange(n), np.diff(off))
+
+    def loss(tt, vv, k):
+        # only the entries K_ij of the PCF i owning point k change: the loss
+        # difference is taken over them, so the rounding of the other
+        # 8.4e6 terms does not enter the difference quotient
+        i = int(owner[k])
+        j = np.arange(n, dtype=np.int64)
+        idx = np.where(j <= i, i * (i + 1) // 2 + j, j * (j + 1) // 2 + i)
+        idx_d = torch.from_numpy(idx).cuda()
+        with torch.no_grad():
+            out = autograd.l2_kernel(torch.tensor(vv, dtype=torch.float64, device="cuda"),
+                                     torch.tensor(tt, dtype=torch.float64, device="cuda"),
+                                     off_d, precision="fp64")
+            return float(torch.dot(Wd[idx_d], out[idx_d]).item())
+
+    rng = np.random.default_rng(13)
+    last = set((off[1:] - 1).tolist())
+    h = 1e-6
+    for k in [k for k in rng.choice(len(v), 12, replace=False) if k not in last]:
+        vp, vm = v.copy(), v.copy()
+        vp[k] += h
+        vm[k] -= h
+        fd = (loss(t, vp, k) - loss(t, vm, k)) / (2 * h)
+        assert gv[k] == pytest.approx(fd, rel=1e-5, abs=1e-6)
+    ts = np.sort(t)
+    interior = np.nonzero(_interior_times(off))[0]
+    checked = 0
+    for k in rng.choice(interior, 40, replace=False):
+        pos = np.searchsorted(ts, t[k])
+        gap = min(ts[pos] - ts[pos - 1] if pos > 0 else np.inf,
+                  ts[pos + 1] - ts[pos] if pos + 1 < len(ts) else np.inf)
+        if gap < 1e-4:
+            continue
+        tp, tm = t.copy(), t.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (loss(tp, v, k) - loss(tm, v, k)) / (2 * h)
+        assert gt[k] == pytest.approx(fd, rel=1e-5, abs=1e-6)
+        checked += 1
+        if checked == 12:
+            break
+    assert checked >= 6
+
+
+def test_cached_set_keeps_each_forwards_values(gpu):
+    """The set and plan of one (offsets, times) pair are cached and their
+    values refreshed on the device: two forwards with different values share
+    them, and each backward still differentiates its own forward."""
+    autograd.clear_cache()
+    off, t, v, w = _inputs(120, seed=21)
+    v2 = v[::-1].copy()
+    v2[off[1:] - 1] = 0.0
+    tt = torch.tensor(t, dtype=torch.float64, device="cuda", requires_grad=True)
+    va = torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True)
+    vb = torch.tensor(v2, dtype=torch.float64, device="cuda", requires_grad=True)
+    off_d = torch.tensor(off, dtype=torch.int64, device="cuda")
+    W = torch.tensor(w, dtype=torch.float64, device="cuda")
+    da = autograd.pdist_l2(va, tt, off_d, precision="fp64")
+    db = autograd.pdist_l2(vb, tt, off_d, precision="fp64")
+    assert len(autograd._cache) == 1 and len(next(iter(autograd._cache.values()))) == 1
+    ((W * da).sum() + 2.0 * (W * db).sum()).backward()
+    for vv, scale, d, g in ((v, 1.0, da, va.grad), (v2, 2.0, db, vb.grad)):
+        fwd = oracle.pairwise(oracle.OP_LP, 2.0, 0, off, t, vv)
+        np.testing.assert_allclose(d.detach().cpu().numpy(), fwd, rtol=1e-13)
+        ev, _ = oracle.grad(oracle.OP_LP, oracle.KIND_PDIST, off, t, vv, scale * w, fwd)
+        assert np.max(np.abs(g.cpu().numpy() - ev)) <= 1e-9 * np.abs(ev).max()
+    _, et_a = oracle.grad(oracle.OP_LP, oracle.KIND_PDIST, off, t, v, w,
+                          oracle.pairwise(oracle.OP_LP, 2.0, 0, off, t, v))
+    _, et_b = oracle.grad(oracle.OP_LP, oracle.KIND_PDIST, off, t, v2, 2.0 * w,
+                          oracle.pairwise(oracle.OP_LP, 2.0, 0, off, t, v2))
+    m = _interior_times(off)
+    et = et_a + et_b
+    assert np.max(np.abs(tt.grad.cpu().numpy()[m] - et[m])) <= 1e-9 * np.abs(et).max()

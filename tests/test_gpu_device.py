@@ -86,6 +86,54 @@ def test_graph_replays_match_eager(gpu):
         _ = torch.randn(1000, device="cuda").sum().item()  # eager work between replays
 
 
+def test_stale_graph_is_refused(gpu):
+    """A graph binds its plan's tile list and launch parameters: after the
+    plan is reshaped, retuned or closed, launching it raises instead of
+    reading freed buffers; a new capture works."""
+    off, t, v = workloads.c2(120, seed=18)
+    exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_PDIST, off, t, v)
+    ds, pl = _plan(off, t, v)
+    out = torch.zeros(pl.out_count, dtype=torch.float64, device="cuda")
+    g = pl.graph(out.data_ptr(), True)
+    pl.set_shape(8, 8, 16, 1)
+    with pytest.raises(ValueError, match="stale"):
+        g.launch(torch.cuda.current_stream().cuda_stream)
+    g2 = pl.graph(out.data_ptr(), True)
+    g2.launch(torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), exp)
+    pl.set_persistent(True)
+    with pytest.raises(ValueError, match="stale"):
+        g2.launch(torch.cuda.current_stream().cuda_stream)
+    g3 = pl.graph(out.data_ptr(), True)
+    pl.close()
+    with pytest.raises(ValueError, match="stale"):
+        g3.launch(torch.cuda.current_stream().cuda_stream)
+
+
+def test_batched_plan_sees_value_updates(gpu):
+    """Batched plans copy their members' records; value updates of a member
+    set reach both eager executes and replays of an earlier capture."""
+    probs = [workloads.c3_problem(b, n=int(n)) for b, n in enumerate([40, 75, 9])]
+    bp = BatchedPlan(probs, op="l1", precision="fp64")
+    out = torch.zeros(bp.out_count, dtype=torch.float64, device="cuda")
+    g = bp.graph(out.data_ptr(), True)
+    off, t, v = probs[1]
+    v2 = v * 0.5 + 3.0
+    v2[off[1:] - 1] = 0.0
+    d_v2 = torch.from_numpy(v2).cuda()
+    bp.sets[1].update_values(d_v2.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    exps = [oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_PDIST, *pr) for pr in probs]
+    exps[1] = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_PDIST, off, t, v2)
+    for launch in (lambda: g.launch(torch.cuda.current_stream().cuda_stream),
+                   lambda: bp.execute(out.data_ptr(), True, torch.cuda.current_stream().cuda_stream)):
+        out.fill_(-1.0)
+        launch()
+        torch.cuda.synchronize()
+        for got, exp in zip(bp.split(out.cpu().numpy()), exps):
+            np.testing.assert_array_equal(got, exp)
+
+
 def test_batched_persistent_graph_matches_oracle(gpu):
     probs = [workloads.c3_problem(b, n=int(n)) for b, n in enumerate([70, 5, 133, 1, 64, 200, 2])]
     bp = BatchedPlan(probs, op="l1", precision="fp32c")

@@ -125,3 +125,33 @@ def test_c5_shape_cdist_rows_and_means(gpu):
                               np.array([0, b - a]), xt[a:b], xv[a:b], *Y)
         got = D[i].double().cpu().numpy()
         assert np.max(np.abs(got - exp) / exp) <= 1e-6
+
+
+def test_c5_full_cdist_rows_past_2_31_and_means(gpu):
+    """The full 50,000 x 50,000 L1 cdist (m ~ U{500..5000}, FP32C with the
+    FP64 fold, 2.5e9 outputs, so output indices pass 2^31): fused row means
+    against the means of the written rows of D, and whole rows at the start
+    and at the end of the index range against the reference walk."""
+    n = 50000
+    X, Y = workloads.c5(n)
+    sx = runtime.DeviceSet(runtime.csr_from_arrays(*X), "fp32c", pack="gpu")
+    sy = runtime.DeviceSet(runtime.csr_from_arrays(*Y), "fp32c", pack="gpu")
+    plan = runtime.Plan(sx, sy, op="l1")
+    assert plan.out_count == n * n > 2 ** 31
+    out = torch.empty(plan.out_count, dtype=torch.float32, device="cuda")
+    rs = torch.empty(n, dtype=torch.float64, device="cuda")
+    plan.execute(out.data_ptr(), False, rs.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    D = out.view(n, n)
+    means = torch.empty(n, dtype=torch.float64, device="cuda")
+    for r0 in range(0, n, 5000):
+        means[r0:r0 + 5000] = D[r0:r0 + 5000].double().mean(dim=1)
+    np.testing.assert_allclose(rs.cpu().numpy() / n, means.cpu().numpy(), rtol=1e-7)
+    assert bool((D[-1] > 0).all().item()) and bool(torch.isfinite(D[-3:]).all().item())
+    xo, xt, xv = X
+    for i in (0, 1, 42950, n - 2, n - 1):  # row 42950 starts past index 2^31
+        a, b = xo[i], xo[i + 1]
+        exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_CDIST,
+                              np.array([0, b - a]), xt[a:b], xv[a:b], *Y)
+        got = D[i].double().cpu().numpy()
+        assert np.max(np.abs(got - exp) / exp) <= 1e-6, i
