@@ -241,6 +241,18 @@ int sbamd_plan_backward(sbamd_plan* plan, const void* grad_out_device, const voi
                         int32_t out_dtype, void* grad_v_device, void* grad_t_device,
                         int32_t grad_dtype, void* stream);
 
+/* The same backward with no device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under hipGraph capture,
+ * tight training or serving loops). Its working arrays live in one static
+ * device array of about 50 MiB per GPU, held until the process ends, sized
+ * for sets of at most 65,536 PCFs and 524,288 records (larger sets:
+ * SBAMD_ERR_INVALID). The plan itself is made beforehand. Every call on a GPU
+ * shares that GPU's array (each clears what it accumulates into): calls
+ * that overlap on one GPU, on any streams, race. */
+int sbamd_plan_backward_noalloc(sbamd_plan* plan, const void* grad_out_device, const void* fwd_out_device,
+                                int32_t out_dtype, void* grad_v_device, void* grad_t_device,
+                                int32_t grad_dtype, void* stream);
+
 /* ---- hipGraph capture of a plan's forward launch ----
  * The graph binds out_device, the plan's tile list and launch parameters and
  * the sets' device arrays. sbamd_graph_launch refuses (SBAMD_ERR_INVALID) a

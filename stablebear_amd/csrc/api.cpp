@@ -18,6 +18,9 @@
 #include <type_traits>
 #include <unordered_map>
 
+// Working memory of sbamd_plan_backward_noalloc (layout in NOALLOC_* below).
+static __device__ __align__(256) unsigned char scratch[(size_t(1) << 19) * 100 + (size_t(1) << 16) * 8];
+
 namespace sbamd
 {
   [[noreturn]] void fail(int code, const std::string& msg) { throw Error{code, msg}; }
@@ -103,6 +106,40 @@ namespace sbamd
     // (legacy default) stream, as everywhere in HIP. The context's own stream
     // is available through sbamd_ctx_stream().
     hipStream_t pick_stream(const Ctx*, void* stream) { return static_cast<hipStream_t>(stream); }
+
+    // Working memory of sbamd_plan_backward_noalloc: one static device array
+    // (every GPU the library runs on holds its own copy until the process
+    // ends), laid out for a set of at most NOALLOC_MAX_PCFS PCFs and
+    // NOALLOC_MAX_RECORDS records: per record start time, value, source
+    // point, two merge-sort keys, Q, A (8 bytes each), one 32-byte event,
+    // owner and two merge-sort ids (4 bytes each); per PCF its weight sum.
+    constexpr int64_t NOALLOC_MAX_RECORDS = int64_t(1) << 19;
+    constexpr int64_t NOALLOC_MAX_PCFS = int64_t(1) << 16;
+    constexpr size_t NOALLOC_KEY0 = 0;
+    constexpr size_t NOALLOC_VAL0 = NOALLOC_KEY0 + 8 * NOALLOC_MAX_RECORDS;
+    constexpr size_t NOALLOC_SRC0 = NOALLOC_VAL0 + 8 * NOALLOC_MAX_RECORDS;
+    constexpr size_t NOALLOC_KEYS0 = NOALLOC_SRC0 + 8 * NOALLOC_MAX_RECORDS;
+    constexpr size_t NOALLOC_KEYS1 = NOALLOC_KEYS0 + 8 * NOALLOC_MAX_RECORDS;
+    constexpr size_t NOALLOC_Q = NOALLOC_KEYS1 + 8 * NOALLOC_MAX_RECORDS;
+    constexpr size_t NOALLOC_A = NOALLOC_Q + 8 * NOALLOC_MAX_RECORDS;
+    constexpr size_t NOALLOC_EVENTS = NOALLOC_A + 8 * NOALLOC_MAX_RECORDS;
+    constexpr size_t NOALLOC_COL0 = NOALLOC_EVENTS + sizeof(BwdEvent) * NOALLOC_MAX_RECORDS;
+    constexpr size_t NOALLOC_IDS0 = NOALLOC_COL0 + 4 * NOALLOC_MAX_RECORDS;
+    constexpr size_t NOALLOC_IDS1 = NOALLOC_IDS0 + 4 * NOALLOC_MAX_RECORDS;
+    constexpr size_t NOALLOC_W = NOALLOC_IDS1 + 4 * NOALLOC_MAX_RECORDS;
+    constexpr size_t NOALLOC_BYTES = NOALLOC_W + 8 * NOALLOC_MAX_PCFS;
+    static_assert(sizeof(BwdEvent) == 32, "event layout");
+    static_assert(NOALLOC_BYTES == sizeof(scratch), "working memory layout");
+
+    // Host code never passes the array's name to the device: it is a
+    // host-side stand-in there. Each device's address of it comes from
+    // hipGetSymbolAddress, once per device, with that device current.
+    std::mutex g_scratch_mu;
+    std::unordered_map<int, void*>& scratch_addr()
+    {
+      static std::unordered_map<int, void*> m;
+      return m;
+    }
 
     // Live plans and their launch-state generations, so that a captured
     // graph can tell whether its plan still holds the buffers and
@@ -659,6 +696,116 @@ namespace sbamd
       }
       return static_cast<uint64_t>(t.nr) * static_cast<uint64_t>(t.nc);
     }
+
+    // Working arrays of one backward run (kernels_bwd.hip).
+    struct BwdWork
+    {
+      double* key0;
+      double* val0;
+      int32_t* col0;
+      int64_t* src0;
+      double* keys[2];
+      int32_t* ids[2];
+      BwdEvent* events;
+      double* q;
+      double* a;
+      double* w;
+    };
+
+    int backward_checks(const Plan* plan, const void* grad_out_device, const void* fwd_out_device,
+                        int32_t out_dtype, void* grad_v_device, void* grad_t_device, int32_t grad_dtype)
+    {
+      need(plan, "plan");
+      need(grad_v_device, "grad_v_device");
+      need(grad_t_device, "grad_t_device");
+      check_out_dtype(out_dtype);
+      check_out_dtype(grad_dtype);
+      if (plan->cdist || plan->batched)
+        fail(SBAMD_ERR_INVALID, "backward supports single-set pdist / l2_kernel plans");
+      const int kop = kernel_op(plan->op, plan->p);
+      if (!((kop == OPK_L2 && plan->out_kind == SBAMD_OUT_DISTMAT) ||
+            (kop == OPK_DOT && plan->out_kind == SBAMD_OUT_SYMMAT)))
+        fail(SBAMD_ERR_INVALID, "backward supports L2 pdist (p = 2) and the L2 kernel");
+      if (!plan->tt_f64)
+        fail(SBAMD_ERR_INVALID, "backward needs a float64 PCF set");
+      if (plan->out_count > 0)
+      {
+        need(grad_out_device, "grad_out_device");
+        if (kop == OPK_L2)
+          need(fwd_out_device, "fwd_out_device");
+      }
+      if (plan->sets[0]->records > INT32_MAX)
+        fail(SBAMD_ERR_INVALID, "backward supports sets of up to 2^31 - 1 records");
+      return kop;
+    }
+
+    // The backward launches on working arrays `wk` (no allocation here).
+    void run_backward(const Plan& pl, int kop, const BwdWork& wk, const void* grad_out_device,
+                      const void* fwd_out_device, int32_t out_dtype, void* grad_v_device,
+                      void* grad_t_device, int32_t grad_dtype, hipStream_t st)
+    {
+      const Set* s = pl.sets[0];
+      const size_t gb = grad_dtype == SBAMD_F64 ? 8 : 4;
+      const size_t M = static_cast<size_t>(s->records);
+      const size_t N = static_cast<size_t>(s->n);
+      {
+        // the gradients and the backward's accumulators, cleared by one launch
+        void* const ptrs[5] = {grad_v_device, grad_t_device, wk.q, wk.a, wk.w};
+        const size_t bytes[5] = {static_cast<size_t>(s->points) * gb, static_cast<size_t>(s->points) * gb,
+                                 M * 8, M * 8, N * 8};
+        launch_fill_zero_multi(ptrs, bytes, 5, st);
+      }
+      BwdParams B;
+      std::memset(&B, 0, sizeof(B));
+      B.n = s->n;
+      B.records = s->records;
+      B.op = kop;
+      B.in_f64 = out_dtype == SBAMD_F64 ? 1 : 0;
+      B.w_f64 = pl.tv_f64; // FP64 sets keep FP64 pair weights
+      B.val_f64 = pl.tv_f64;
+      B.grad_f64 = grad_dtype == SBAMD_F64 ? 1 : 0;
+      // chunks of 64 partners: 16 KB (FP32) or 32 KB (FP64) of weights in
+      // LDS, so that 16-wave workgroups still fill a CU
+      B.ch = BWD_GROUP;
+      B.roff = s->roff.as<int64_t>();
+      B.rec = s->rec.ptr;
+      B.perm = s->perm.as<int64_t>();
+      B.src = s->src.as<int64_t>();
+      B.ooff = pl.bwd_ooff.as<int64_t>();
+      B.key0 = wk.key0;
+      B.val0 = wk.val0;
+      B.col0 = wk.col0;
+      B.src0 = wk.src0;
+      for (int b = 0; b < 2; ++b)
+      {
+        B.keys[b] = wk.keys[b];
+        B.ids[b] = wk.ids[b];
+      }
+      B.events = wk.events;
+      B.events2 = B.events; // chunks are the same groups of 64
+      B.grad_out = grad_out_device;
+      B.fwd_out = fwd_out_device;
+      B.q = wk.q;
+      B.a = wk.a;
+      B.wsum = wk.w;
+      B.grad_v = grad_v_device;
+      B.grad_t = grad_t_device;
+      launch_backward(B, st);
+    }
+
+    // Record offsets of a single-set plan's PCFs in the original order, for
+    // the backward's position layout.
+    void upload_ooff(Plan& pl)
+    {
+      const Set* s = pl.sets[0];
+      const size_t N = static_cast<size_t>(s->n);
+      std::vector<int64_t> ooff(N + 1, 0);
+      for (size_t q = 0; q < N; ++q)
+        ooff[static_cast<size_t>(s->h_perm[q]) + 1] = s->h_roff[q + 1] - s->h_roff[q];
+      for (size_t i = 0; i < N; ++i)
+        ooff[i + 1] += ooff[i];
+      upload(pl.bwd_ooff, ooff);
+    }
   }
 
   void Plan::bump()
@@ -779,6 +926,19 @@ extern "C"
       SBAMD_HIP(hipGetDeviceProperties(&prop, device));
       ctx->cu_count = prop.multiProcessorCount;
       ctx->lds_per_block = prop.sharedMemPerBlock;
+      {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        auto it = scratch_addr().find(device);
+        if (it == scratch_addr().end())
+        {
+          void* ptr = nullptr;
+          SBAMD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+          if (!ptr)
+            fail(SBAMD_ERR_RUNTIME, "hipGetSymbolAddress returned no address for the working memory");
+          it = scratch_addr().emplace(device, ptr).first;
+        }
+        ctx->scratch = it->second;
+      }
       SBAMD_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
       *out = ctx.release();
     });
@@ -1095,6 +1255,8 @@ extern "C"
       check_shape(pl->R, pl->C, pl->cap, pl->ppl);
       pl->rebuild_tiles();
       decide_long(*pl);
+      if (!pl->cdist)
+        upload_ooff(*pl);
       *out = pl.release();
     });
   }
@@ -1385,42 +1547,14 @@ extern "C"
                           int32_t grad_dtype, void* stream)
   {
     return guarded([&] {
-      need(plan, "plan");
-      need(grad_v_device, "grad_v_device");
-      need(grad_t_device, "grad_t_device");
-      check_out_dtype(out_dtype);
-      check_out_dtype(grad_dtype);
-      if (plan->cdist || plan->batched)
-        fail(SBAMD_ERR_INVALID, "backward supports single-set pdist / l2_kernel plans");
-      const int kop = kernel_op(plan->op, plan->p);
-      if (!((kop == OPK_L2 && plan->out_kind == SBAMD_OUT_DISTMAT) ||
-            (kop == OPK_DOT && plan->out_kind == SBAMD_OUT_SYMMAT)))
-        fail(SBAMD_ERR_INVALID, "backward supports L2 pdist (p = 2) and the L2 kernel");
-      if (!plan->tt_f64)
-        fail(SBAMD_ERR_INVALID, "backward needs a float64 PCF set");
-      if (plan->out_count > 0)
-      {
-        need(grad_out_device, "grad_out_device");
-        if (kop == OPK_L2)
-          need(fwd_out_device, "fwd_out_device");
-      }
+      const int kop = backward_checks(plan, grad_out_device, fwd_out_device, out_dtype, grad_v_device,
+                                      grad_t_device, grad_dtype);
       use_device(plan->ctx);
-      hipStream_t st = pick_stream(plan->ctx, stream);
       const Set* s = plan->sets[0];
-      if (s->records > INT32_MAX)
-        fail(SBAMD_ERR_INVALID, "backward supports sets of up to 2^31 - 1 records");
-      const size_t gb = grad_dtype == SBAMD_F64 ? 8 : 4;
       const size_t M = static_cast<size_t>(s->records);
       const size_t N = static_cast<size_t>(s->n);
       if (plan->bwd_q.bytes < M * 8)
       {
-        // record offsets in the original PCF order
-        std::vector<int64_t> ooff(N + 1, 0);
-        for (size_t q = 0; q < N; ++q)
-          ooff[static_cast<size_t>(s->h_perm[q]) + 1] = s->h_roff[q + 1] - s->h_roff[q];
-        for (size_t i = 0; i < N; ++i)
-          ooff[i + 1] += ooff[i];
-        upload(plan->bwd_ooff, ooff);
         plan->bwd_key0 = DeviceBuffer(M * 8);
         plan->bwd_val0 = DeviceBuffer(M * 8);
         plan->bwd_col0 = DeviceBuffer(M * 4);
@@ -1435,50 +1569,58 @@ extern "C"
         plan->bwd_a = DeviceBuffer(M * 8);
         plan->bwd_w = DeviceBuffer(N * 8);
       }
-      {
-        // the gradients and the backward's accumulators, cleared by one launch
-        void* const ptrs[5] = {grad_v_device, grad_t_device, plan->bwd_q.ptr, plan->bwd_a.ptr,
-                               plan->bwd_w.ptr};
-        const size_t bytes[5] = {static_cast<size_t>(s->points) * gb, static_cast<size_t>(s->points) * gb,
-                                 M * 8, M * 8, N * 8};
-        launch_fill_zero_multi(ptrs, bytes, 5, st);
-      }
-      BwdParams B;
-      std::memset(&B, 0, sizeof(B));
-      B.n = s->n;
-      B.records = s->records;
-      B.op = kop;
-      B.in_f64 = out_dtype == SBAMD_F64 ? 1 : 0;
-      B.w_f64 = plan->tv_f64; // FP64 sets keep FP64 pair weights
-      B.val_f64 = plan->tv_f64;
-      B.grad_f64 = grad_dtype == SBAMD_F64 ? 1 : 0;
-      // chunks of 64 partners: 16 KB (FP32) or 32 KB (FP64) of weights in
-      // LDS, so that 16-wave workgroups still fill a CU
-      B.ch = BWD_GROUP;
-      B.roff = s->roff.as<int64_t>();
-      B.rec = s->rec.ptr;
-      B.perm = s->perm.as<int64_t>();
-      B.src = s->src.as<int64_t>();
-      B.ooff = plan->bwd_ooff.as<int64_t>();
-      B.key0 = plan->bwd_key0.as<double>();
-      B.val0 = plan->bwd_val0.as<double>();
-      B.col0 = plan->bwd_col0.as<int32_t>();
-      B.src0 = plan->bwd_src0.as<int64_t>();
+      BwdWork wk;
+      wk.key0 = plan->bwd_key0.as<double>();
+      wk.val0 = plan->bwd_val0.as<double>();
+      wk.col0 = plan->bwd_col0.as<int32_t>();
+      wk.src0 = plan->bwd_src0.as<int64_t>();
       for (int b = 0; b < 2; ++b)
       {
-        B.keys[b] = plan->bwd_keys[b].as<double>();
-        B.ids[b] = plan->bwd_ids[b].as<int32_t>();
+        wk.keys[b] = plan->bwd_keys[b].as<double>();
+        wk.ids[b] = plan->bwd_ids[b].as<int32_t>();
       }
-      B.events = plan->bwd_events.as<BwdEvent>();
-      B.events2 = B.events; // chunks are the same groups of 64
-      B.grad_out = grad_out_device;
-      B.fwd_out = fwd_out_device;
-      B.q = plan->bwd_q.as<double>();
-      B.a = plan->bwd_a.as<double>();
-      B.wsum = plan->bwd_w.as<double>();
-      B.grad_v = grad_v_device;
-      B.grad_t = grad_t_device;
-      launch_backward(B, st);
+      wk.events = plan->bwd_events.as<BwdEvent>();
+      wk.q = plan->bwd_q.as<double>();
+      wk.a = plan->bwd_a.as<double>();
+      wk.w = plan->bwd_w.as<double>();
+      run_backward(*plan, kop, wk, grad_out_device, fwd_out_device, out_dtype, grad_v_device, grad_t_device,
+                   grad_dtype, pick_stream(plan->ctx, stream));
+    });
+  }
+
+  int sbamd_plan_backward_noalloc(sbamd_plan* plan, const void* grad_out_device, const void* fwd_out_device,
+                                  int32_t out_dtype, void* grad_v_device, void* grad_t_device,
+                                  int32_t grad_dtype, void* stream)
+  {
+    return guarded([&] {
+      const int kop = backward_checks(plan, grad_out_device, fwd_out_device, out_dtype, grad_v_device,
+                                      grad_t_device, grad_dtype);
+      const Set* s = plan->sets[0];
+      if (s->records > NOALLOC_MAX_RECORDS || s->n > NOALLOC_MAX_PCFS)
+        fail(SBAMD_ERR_INVALID, "sbamd_plan_backward_noalloc: the set has " + std::to_string(s->n) +
+                                    " PCFs and " + std::to_string(s->records) +
+                                    " records; its working memory holds at most " +
+                                    std::to_string(NOALLOC_MAX_PCFS) + " PCFs and " +
+                                    std::to_string(NOALLOC_MAX_RECORDS) + " records");
+      use_device(plan->ctx);
+      unsigned char* const base = static_cast<unsigned char*>(plan->ctx->scratch);
+      if (!base)
+        fail(SBAMD_ERR_RUNTIME, "no working memory address for this device");
+      BwdWork wk;
+      wk.key0 = reinterpret_cast<double*>(base + NOALLOC_KEY0);
+      wk.val0 = reinterpret_cast<double*>(base + NOALLOC_VAL0);
+      wk.src0 = reinterpret_cast<int64_t*>(base + NOALLOC_SRC0);
+      wk.keys[0] = reinterpret_cast<double*>(base + NOALLOC_KEYS0);
+      wk.keys[1] = reinterpret_cast<double*>(base + NOALLOC_KEYS1);
+      wk.q = reinterpret_cast<double*>(base + NOALLOC_Q);
+      wk.a = reinterpret_cast<double*>(base + NOALLOC_A);
+      wk.events = reinterpret_cast<BwdEvent*>(base + NOALLOC_EVENTS);
+      wk.col0 = reinterpret_cast<int32_t*>(base + NOALLOC_COL0);
+      wk.ids[0] = reinterpret_cast<int32_t*>(base + NOALLOC_IDS0);
+      wk.ids[1] = reinterpret_cast<int32_t*>(base + NOALLOC_IDS1);
+      wk.w = reinterpret_cast<double*>(base + NOALLOC_W);
+      run_backward(*plan, kop, wk, grad_out_device, fwd_out_device, out_dtype, grad_v_device, grad_t_device,
+                   grad_dtype, pick_stream(plan->ctx, stream));
     });
   }
 

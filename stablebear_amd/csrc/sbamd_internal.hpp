@@ -230,6 +230,7 @@ namespace sbamd
     hipStream_t stream = nullptr;
     int cu_count = 0;
     size_t lds_per_block = 0;
+    void* scratch = nullptr; // this device's address of the no-alloc working memory
   };
 
   struct Set

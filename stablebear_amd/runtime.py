@@ -460,6 +460,13 @@ class Plan:
                                             L.F64 if out_f64 else L.F32, grad_v_ptr, grad_t_ptr,
                                             L.F64 if grad_f64 else L.F32, stream))
 
+    def backward_noalloc(self, grad_out_ptr, fwd_out_ptr, out_f64, grad_v_ptr, grad_t_ptr,
+                         grad_f64, stream=None):
+        """``backward`` with no device allocation (a static working array)."""
+        L.check(L.lib().sbamd_plan_backward_noalloc(self.handle, grad_out_ptr, fwd_out_ptr,
+                                                    L.F64 if out_f64 else L.F32, grad_v_ptr,
+                                                    grad_t_ptr, L.F64 if grad_f64 else L.F32, stream))
+
     def autotune(self, out_ptr: int, out_f64: bool, reps: int = 3):
         sh = L.TileShape()
         ms = ctypes.c_double()
