@@ -245,6 +245,15 @@ def main(argv=None):
                          "m ~ U{500..5000}, FP64) packed into a device set on the host and on "
                          "the GPU")
     ap.add_argument("--ingest-n", type=int, default=50000)
+    ap.add_argument("--quick", action="store_true",
+                    help="only the C2 legs (skip C1, C3, C5 and the end-to-end leg)")
+    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true")
+    ap.add_argument("--e2e", action="store_true",
+                    help="run only the end-to-end leg (sb.pdist from Pcf objects to a filled matrix)")
+    ap.add_argument("--c1", action="store_true", help="run only the C1 leg")
     ap.add_argument("--long", action="store_true",
                     help="run only the long-pair leg: L1 pdist of 64 PCFs with m ~ U{50k..100k}, "
                          "tile kernel vs long-pair kernel")
@@ -285,6 +294,12 @@ def main(argv=None):
         return 0
     if args.norms:
         print(json.dumps(_norms_leg(args, stream)), flush=True)
+        return 0
+    if args.e2e:
+        print(json.dumps(_e2e_leg(args)), flush=True)
+        return 0
+    if args.c1:
+        print(json.dumps(_c1_leg(stream)), flush=True)
         return 0
 
     n = args.n
@@ -359,6 +374,20 @@ def main(argv=None):
     if not args.no_ops:
         ops = _other_ops(csr, off, t, v, n, max(1, min(args.steps, 3)), stream, cores, 300_000)
         _progress("L2 pdist / L2 kernel done")
+    extra = {}
+    if not args.quick:
+        if not args.no_c1:
+            extra["c1"] = _c1_leg(stream)
+            _progress(f"C1: {extra['c1']['fp64']['ms']:.3f} ms")
+        if not args.no_e2e:
+            extra["e2e"] = _e2e_leg(args)
+            _progress(f"end to end: {extra['e2e']}")
+        if not args.no_c3:
+            extra["c3_batched"] = _batched_leg(args, stream, cpu_seconds=5.0)
+            _progress(f"C3: graph {extra['c3_batched']['graph_ms']:.2f} ms")
+        if not args.no_c5:
+            extra["c5_cdist"] = _cdist_leg(args, stream)
+            _progress(f"C5: {extra['c5_cdist']['ms']:.1f} ms")
     fwdbwd = None
     if not args.no_fwdbwd:
         fwdbwd = _fwd_bwd_c4(stream)
@@ -399,6 +428,7 @@ def main(argv=None):
         line["ops"] = ops
     if fwdbwd is not None:
         line["fwd_bwd"] = fwdbwd
+    line.update(extra)
     print(json.dumps(line), flush=True)
     return 0
 
@@ -481,7 +511,86 @@ def _fwd_bwd_c4(stream):
     return res
 
 
-def _batched_leg(args, stream):
+def _roofline(flops_per_segment, segments, ms, fp64=False):
+    """Algorithmic FLOP/s of a launch against the MI355X vector peak."""
+    achieved = flops_per_segment * segments / (ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS["fp64" if fp64 else "fp32"]
+    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak}
+
+
+def _c1_leg(stream):
+    """C1: L1 pdist of N = 1,000 PCFs with m ~ U{10..100}, values m-1..0 (integers),
+    FP64 and FP32C on the device against the whole oracle matrix, and the
+    reference walk on the host cores over the whole matrix."""
+    import torch
+
+    import oracle
+    from stablebear_amd import runtime, workloads
+
+    off, t, v = workloads.c1()
+    csr = runtime.csr_from_arrays(off, t, v)
+    cores = len(os.sched_getaffinity(0))
+    use_ref = oracle.ref_available()
+    t0 = time.perf_counter()
+    exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_PDIST, off, t, v, nthreads=cores,
+                          use_ref=use_ref)
+    cpu_s = time.perf_counter() - t0
+    res = {"workload": "L1 pdist, N=1000, m~U{10..100}, values m-1..0"}
+    for prec in ("fp64", "fp32c"):
+        ds = runtime.DeviceSet(csr, prec)
+        plan = runtime.Plan(ds, op="l1")
+        f64 = prec == "fp64"
+        out = torch.empty(plan.out_count, dtype=torch.float64 if f64 else torch.float32,
+                          device="cuda")
+        ms, _ = _events_ms(lambda: plan.execute(out.data_ptr(), f64, stream=stream.cuda_stream),
+                           20, 3, stream)
+        got = out.double().cpu().numpy()
+        rel, _ = _errors(got, exp)
+        res[prec] = {"ms": ms, "pairs_per_s": plan.pairs / (ms * 1e-3), "max_rel_err": rel,
+                     "bitexact": bool(np.array_equal(got, exp)) if f64 else None,
+                     "roofline": _roofline(FLOPS_PER_SEGMENT["l1"], plan.segments, ms, f64)}
+    res["pairs"] = int(len(exp))
+    res["cpu_baseline"] = {"value": len(exp) / cpu_s, "unit": "pairs/s", "cores": cores,
+                           "kind": "reference" if use_ref else "port",
+                           "sample": f"the whole pdist ({len(exp)} pairs, {cpu_s:.3f} s)"}
+    return res
+
+
+def _e2e_leg(args):
+    """The drop-in end to end on the C2 workload: sb.pdist on a list of 20,000
+    Pcf objects until the DistanceMatrix is filled (host CSR assembly, set
+    upload and packing, planning, the kernels, the banded download into the
+    matrix storage), wall clock, FP32C and FP64; a sample of entries is
+    checked against the oracle."""
+    import stablebear_amd as sb
+    import oracle
+    from stablebear_amd import workloads
+
+    n = args.n
+    off, t, v = workloads.c2(n)
+    pcfs = workloads.to_pcfs(off, t, v)
+    res = {"workload": f"sb.pdist(list of {n} Pcf, L1) -> DistanceMatrix, m~U{{50..500}}"}
+    r1 = 2
+    exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_PDIST, off, t, v, rows=(0, r1))
+    idx = _storage_rows(n, r1)
+    for prec in ("fp32c", "fp64"):
+        sb.pdist(pcfs[:300], precision=prec)  # warm the library and the packers
+        best = None
+        for _ in range(2):
+            t0 = time.perf_counter()
+            D = sb.pdist(pcfs, precision=prec)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        got = np.asarray(D.storage, dtype=np.float64)[idx]
+        rel, _ = _errors(got, exp[idx])
+        res[prec] = {"ms": best * 1e3, "pairs_per_s": n * (n - 1) / 2 / best,
+                     "max_rel_err_sample": rel}
+        del D
+    return res
+
+
+def _batched_leg(args, stream, cpu_seconds=10.0):
     """256 independent pdist problems (N_b ~ U{64..4096}, m ~ round(LogNormal(4, 1))
     clipped to [5, 20000], integer values in [-50, 50] ending at 0), L1, FP32C: one
     persistent launch over all problems, tile shape picked by the autotuner, captured
@@ -526,7 +635,7 @@ def _batched_leg(args, stream):
         rel, _ = _errors(parts[b].astype(np.float64), exp)
         max_rel = max(max_rel, rel)
         checked += 1
-        if cpu_s > 10.0:
+        if cpu_s > cpu_seconds:
             break
     return {
         "leg": "batched",
@@ -536,6 +645,7 @@ def _batched_leg(args, stream):
         "segments": plan.segments, "tile": list(shape),
         "graph_ms": ms_graph, "eager_ms": ms_eager,
         "pairs_per_s": plan.pairs / (ms_graph * 1e-3),
+        "roofline": _roofline(FLOPS_PER_SEGMENT["l1"], plan.segments, ms_graph),
         "segments_per_s": plan.segments / (ms_graph * 1e-3),
         "setup_s": setup_s,
         "max_rel_err_vs_fp64_reference": max_rel,
@@ -703,6 +813,7 @@ def _cdist_leg(args, stream):
         "pairs": plan.pairs, "segments": plan.segments, "tiles": plan.tiles,
         "tile": list(plan.shape), "ms": ms,
         "pairs_per_s": plan.pairs / (ms * 1e-3),
+        "roofline": _roofline(FLOPS_PER_SEGMENT["l1"], plan.segments, ms),
         "segments_per_s": plan.segments / (ms * 1e-3),
         "setup_s": setup_s,
         "max_rel_err_vs_fp64_reference": rel,

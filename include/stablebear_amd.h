@@ -135,6 +135,15 @@ int sbamd_ctx_destroy(sbamd_ctx* ctx);
 int sbamd_ctx_synchronize(sbamd_ctx* ctx);
 /* The context's own (non-blocking) stream, for callers without a stream. */
 int sbamd_ctx_stream(sbamd_ctx* ctx, void** stream);
+/* Device bytes the host-to-host jobs (sbamd_pairwise_submit) may give their
+ * output at once; 0 (the default) = half the free device memory when the job
+ * starts. Outputs above it, or above 256 MiB, run in bands of rows whose
+ * downloads overlap the next bands' compute. Replaces the reference's output
+ * sizing from free memory (block_matrix_support.cuh:69-93). */
+int sbamd_ctx_set_output_budget(sbamd_ctx* ctx, uint64_t bytes);
+/* Tile shape of the jobs' plans (zero fields: defaults). Replaces the
+ * reference's block-size settings (stablebear/system.py set_block_size). */
+int sbamd_ctx_set_tile_shape(sbamd_ctx* ctx, const sbamd_tile_shape* shape);
 
 /* ---- device-resident PCF sets ----
  * Validates, drops zero-length segments, sorts PCFs by record count and
@@ -272,8 +281,12 @@ int sbamd_autotune(sbamd_plan* plan, void* out_device, int32_t out_dtype, int32_
 int sbamd_plan_set_shape(sbamd_plan* plan, const sbamd_tile_shape* shape);
 
 /* ---- asynchronous host-to-host jobs (the reference's StoppableTask) ----
- * Runs set upload, planning, execution in chunks, and the download into
- * out_host on a worker thread. progress counts pairs ("integral" unit).
+ * Runs set upload, planning, execution and the download into out_host on a
+ * worker thread. Outputs within the context's output budget (and 256 MiB)
+ * are computed whole on the device and downloaded once; larger ones in bands
+ * of consecutive rows (original order), each band's output written in place
+ * on the device, downloaded into pinned memory while the next band computes,
+ * and copied into out_host. progress counts pairs ("integral" unit).
  * Replaces create_cuda_block_{integrate_l1,integrate_lp,integrate_l2_kernel,
  * cdist_l1,cdist_lp}_task (cuda_matrix_integrate_api.hpp:19-113) and the
  * StoppableTask wait / progress / cancel protocol (task.hpp:15-193). */

@@ -123,6 +123,8 @@ _SIGS = {
     "sbamd_plan_set_long_pairs": (None, [vp, ctypes.c_int32]),
     "sbamd_plan_execute": (None, [vp, vp, ctypes.c_int32, vp, vp]),
     "sbamd_plan_backward": (None, [vp, vp, vp, ctypes.c_int32, vp, vp, ctypes.c_int32, vp]),
+    "sbamd_ctx_set_output_budget": (None, [vp, ctypes.c_uint64]),
+    "sbamd_ctx_set_tile_shape": (None, [vp, ctypes.POINTER(TileShape)]),
     "sbamd_plan_backward_noalloc": (None, [vp, vp, vp, ctypes.c_int32, vp, vp, ctypes.c_int32, vp]),
     "sbamd_plan_execute_to_host": (None, [vp, vp, ctypes.c_int32]),
     "sbamd_plain_create": (None, [vp, ctypes.POINTER(PcfSet), ctypes.POINTER(vp)]),

@@ -806,6 +806,114 @@ namespace sbamd
         ooff[i + 1] += ooff[i];
       upload(pl.bwd_ooff, ooff);
     }
+
+    // A plan over rows (and cols: a cross plan). Cross plans normally write a
+    // dense matrix; the host-to-host jobs also make cross plans that write
+    // compressed outputs (pairs whose row index exceeds every column index).
+    // out_base is added to every output index.
+    std::unique_ptr<sbamd_plan> new_plan(Ctx* ctx, const Set* rows, const Set* cols, int32_t op, double p,
+                                         int32_t out_kind, const sbamd_tile_shape* shape, int64_t out_base)
+    {
+      if (op == SBAMD_OP_LP && !(p >= 1.0))
+        fail(SBAMD_ERR_INVALID, "p must be >= 1.");
+      (void)kernel_op(op, p);
+      auto pl = std::make_unique<sbamd_plan>();
+      pl->ctx = ctx;
+      pl->op = op;
+      pl->p = op == SBAMD_OP_LP ? p : 1.0;
+      pl->out_kind = out_kind;
+      pl->precision = rows->precision;
+      pl->tt_f64 = rows->tt_f64;
+      pl->tv_f64 = rows->tv_f64;
+      pl->sets.push_back(rows);
+      if (cols)
+      {
+        if (cols->precision != rows->precision || cols->tt_f64 != rows->tt_f64 ||
+            cols->tv_f64 != rows->tv_f64)
+          fail(SBAMD_ERR_INVALID, "row and column sets must share dtype and precision");
+        pl->cdist = true;
+        pl->sets.push_back(cols);
+        pl->pairs = static_cast<uint64_t>(rows->n) * static_cast<uint64_t>(cols->n);
+        pl->out_count = pl->pairs;
+        pl->segments = static_cast<uint64_t>(cols->n) * rows->records +
+                       static_cast<uint64_t>(rows->n) * cols->records - pl->pairs;
+      }
+      else
+      {
+        const uint64_t n = static_cast<uint64_t>(rows->n);
+        const uint64_t M = static_cast<uint64_t>(rows->records);
+        if (out_kind == SBAMD_OUT_DISTMAT)
+        {
+          pl->pairs = n > 1 ? n * (n - 1) / 2 : 0;
+          pl->segments = n > 1 ? (n - 1) * M - pl->pairs : 0;
+        }
+        else if (out_kind == SBAMD_OUT_SYMMAT)
+        {
+          pl->pairs = n * (n + 1) / 2;
+          pl->segments = n > 0 ? n * M - (n > 1 ? n * (n - 1) / 2 : 0) : 0;
+        }
+        else
+          fail(SBAMD_ERR_INVALID, "pdist / kernel plans need SBAMD_OUT_DISTMAT or SBAMD_OUT_SYMMAT");
+        pl->out_count = pl->pairs;
+      }
+      Problem pr{0, 0, out_base, rows->n};
+      pl->h_probs.push_back(pr);
+      pl->input_bytes = 0;
+      for (const Set* st : pl->sets)
+        pl->input_bytes += st->rec.bytes + st->roff.bytes;
+      if (shape)
+      {
+        if (shape->rows)
+          pl->R = shape->rows;
+        if (shape->cols)
+          pl->C = shape->cols;
+        if (shape->cap)
+          pl->cap = shape->cap;
+        if (shape->lane_pairs)
+          pl->ppl = shape->lane_pairs;
+      }
+      if (!pl->cdist)
+        pl->C = pl->R;
+      check_shape(pl->R, pl->C, pl->cap, pl->ppl);
+      pl->rebuild_tiles();
+      if (out_base != 0)
+        pl->long_mode = 0; // the long-pair kernel writes whole outputs only
+      decide_long(*pl);
+      if (!pl->cdist && out_base == 0)
+        upload_ooff(*pl);
+      return pl;
+    }
+
+    // Subset of a device set: the PCFs at sorted positions `sel` (ascending,
+    // so still sorted by record count), records gathered on the device. For
+    // the forward only (no source points).
+    void make_subset(const Set& full, const std::vector<int64_t>& sel, Set& s, DeviceBuffer& d_sel,
+                     hipStream_t st)
+    {
+      s.ctx = full.ctx;
+      s.precision = full.precision;
+      s.in_dtype = full.in_dtype;
+      s.tt_f64 = full.tt_f64;
+      s.tv_f64 = full.tv_f64;
+      s.n = static_cast<int64_t>(sel.size());
+      s.h_roff.assign(sel.size() + 1, 0);
+      s.h_perm.resize(sel.size());
+      for (size_t k = 0; k < sel.size(); ++k)
+      {
+        const int64_t q = sel[k];
+        s.h_roff[k + 1] = s.h_roff[k] + (full.h_roff[q + 1] - full.h_roff[q]);
+        s.h_perm[k] = full.h_perm[q];
+      }
+      s.records = s.h_roff.back();
+      s.points = 0;
+      upload(s.roff, s.h_roff);
+      upload(s.perm, s.h_perm);
+      upload(d_sel, sel); // kept by the caller until the gather has run
+      const int rb = record_bytes(s.tt_f64);
+      s.rec = DeviceBuffer(static_cast<size_t>(s.records) * rb);
+      launch_gather_subset(full.roff.as<int64_t>(), full.rec.ptr, rb, d_sel.as<int64_t>(), s.roff.as<int64_t>(),
+                           s.n, s.records, s.rec.ptr, st);
+    }
   }
 
   void Plan::bump()
@@ -891,6 +999,305 @@ struct sbamd_plain
   DeviceBuffer off, t, v;
 };
 
+namespace
+{
+  // Host-to-host job bodies (the reference's CudaPairwiseIntegrationTask /
+  // CudaCrossIntegrationTask, pcf_block_op.cuh:127-286, with the block
+  // pipeline of cuda_block_executor.cuh:224-297).
+  constexpr uint64_t JOB_SINGLE_MAX_BYTES = uint64_t(256) << 20;
+
+  template <typename T, typename D>
+  std::unique_ptr<typename std::remove_pointer<T>::type, D> guard(T p, D d)
+  {
+    return std::unique_ptr<typename std::remove_pointer<T>::type, D>(p, d);
+  }
+
+  // memcpy over several host threads: a band's copy out of pinned memory
+  // into the caller's (often freshly allocated, not yet touched) array is
+  // bound by page faults and one core's bandwidth otherwise
+  void parallel_copy(void* dst, const void* src, size_t bytes)
+  {
+    constexpr size_t MIN_PART = size_t(16) << 20;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t parts = std::min<size_t>(std::min<size_t>(hw, 16), std::max<size_t>(1, bytes / MIN_PART));
+    if (parts <= 1)
+    {
+      std::memcpy(dst, src, bytes);
+      return;
+    }
+    std::vector<std::thread> th;
+    const size_t step = (bytes + parts - 1) / parts;
+    for (size_t k = 1; k < parts; ++k)
+    {
+      const size_t a = k * step, b = std::min(bytes, a + step);
+      if (a < b)
+        th.emplace_back([=]() {
+          std::memcpy(static_cast<unsigned char*>(dst) + a, static_cast<const unsigned char*>(src) + a, b - a);
+        });
+    }
+    std::memcpy(dst, src, std::min(bytes, step));
+    for (auto& t : th)
+      t.join();
+  }
+
+  sbamd_tile_shape job_shape(const Ctx* ctx)
+  {
+    sbamd_tile_shape sh;
+    sh.rows = ctx->job_R;
+    sh.cols = ctx->job_C;
+    sh.cap = ctx->job_cap;
+    sh.lane_pairs = ctx->job_ppl;
+    return sh;
+  }
+
+  // Whole output on the device, tile ranges for progress and cancel, one
+  // download at the end (outputs that fit easily).
+  void job_single(Ctx* ctx, sbamd_job* jp, Set& rs, Set* cs, int32_t op, double p, int32_t out_kind,
+                  void* out_host, int32_t out_dtype)
+  {
+    const sbamd_tile_shape sh = job_shape(ctx);
+    auto plan = new_plan(ctx, &rs, cs, op, p, out_kind, &sh, 0);
+    const size_t eb = out_dtype == SBAMD_F64 ? 8 : 4;
+    DeviceBuffer dout(static_cast<size_t>(plan->out_count) * eb);
+    hipStream_t st;
+    SBAMD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    auto sguard = guard(st, &hipStreamDestroy);
+    const int64_t nt = work_units(*plan);
+    const int64_t chunks = std::min<int64_t>(nt, 64);
+    const bool diag = out_kind == SBAMD_OUT_SYMMAT;
+    hipEvent_t ev;
+    SBAMD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    auto eguard = guard(ev, &hipEventDestroy);
+    for (int64_t c = 0; c < chunks; ++c)
+    {
+      if (jp->cancel.load())
+        fail(SBAMD_ERR_CANCELLED, "cancelled");
+      const int64_t t0 = nt * c / chunks, t1 = nt * (c + 1) / chunks;
+      execute_range(*plan, dout.ptr, out_dtype, nullptr, t0, t1, st);
+      SBAMD_HIP(hipEventRecord(ev, st));
+      while (hipEventQuery(ev) == hipErrorNotReady)
+      {
+        if (jp->cancel.load())
+          break;
+        std::this_thread::sleep_for(std::chrono::microseconds(200));
+      }
+      SBAMD_HIP(hipStreamSynchronize(st));
+      uint64_t done = 0;
+      if (plan->use_long)
+        done = static_cast<uint64_t>(t1 - t0);
+      else
+        for (int64_t k = t0; k < t1; ++k)
+          done += tile_pairs(plan->h_tiles[k], diag);
+      jp->progress += done;
+    }
+    SBAMD_HIP(hipMemcpy(out_host, dout.ptr, static_cast<size_t>(plan->out_count) * eb, hipMemcpyDeviceToHost));
+  }
+
+  // Row bands in the original order, each writing one contiguous range of
+  // the output: rows [r0, r1) against every earlier PCF (a cross plan) and
+  // among themselves (a pdist / kernel plan) for the compressed layouts, or
+  // against every column PCF for a dense cdist. Bands alternate between two
+  // device buffers and two pinned host buffers: a band's download (on its
+  // own stream) and its copy into out_host overlap the next bands' compute.
+  void job_bands(Ctx* ctx, sbamd_job* jp, Set& rs, Set* cs, int32_t op, double p, int32_t out_kind,
+                 void* out_host, int32_t out_dtype, uint64_t chunk_elems)
+  {
+    const size_t eb = out_dtype == SBAMD_F64 ? 8 : 4;
+    const int64_t n = rs.n;
+    const int64_t ncols = cs ? cs->n : 0;
+    // output elements of rows [0, r) in the original order
+    auto cum = [&](int64_t r) -> uint64_t {
+      const uint64_t u = static_cast<uint64_t>(r);
+      if (cs)
+        return u * static_cast<uint64_t>(ncols);
+      return out_kind == SBAMD_OUT_SYMMAT ? u * (u + 1) / 2 : (u > 0 ? u * (u - 1) / 2 : 0);
+    };
+    std::vector<int64_t> bounds{0};
+    while (bounds.back() < n)
+    {
+      const int64_t r0 = bounds.back();
+      int64_t lo = r0 + 1, hi = n; // largest r1 with cum(r1) - cum(r0) <= chunk_elems, at least r0 + 1
+      while (lo < hi)
+      {
+        const int64_t mid = lo + (hi - lo + 1) / 2;
+        if (cum(mid) - cum(r0) <= chunk_elems)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      if (cum(lo) - cum(r0) > chunk_elems)
+        fail(SBAMD_ERR_NOMEM, "one output row needs more device memory than the output budget allows");
+      bounds.push_back(lo);
+    }
+    const int64_t nb = static_cast<int64_t>(bounds.size()) - 1;
+    const sbamd_tile_shape sh = job_shape(ctx);
+
+    hipStream_t st, cst;
+    SBAMD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    auto g1 = guard(st, &hipStreamDestroy);
+    SBAMD_HIP(hipStreamCreateWithFlags(&cst, hipStreamNonBlocking));
+    auto g2 = guard(cst, &hipStreamDestroy);
+    const size_t buf_bytes = static_cast<size_t>(chunk_elems) * eb;
+    DeviceBuffer dbuf[2] = {DeviceBuffer(buf_bytes), DeviceBuffer(buf_bytes)};
+    void* hbuf[2] = {nullptr, nullptr};
+    SBAMD_HIP(hipHostMalloc(&hbuf[0], buf_bytes, hipHostMallocDefault));
+    auto h0 = guard(hbuf[0], &hipHostFree);
+    SBAMD_HIP(hipHostMalloc(&hbuf[1], buf_bytes, hipHostMallocDefault));
+    auto h1 = guard(hbuf[1], &hipHostFree);
+    hipEvent_t done_ev[2], copied_ev[2];
+    for (int b = 0; b < 2; ++b)
+    {
+      SBAMD_HIP(hipEventCreateWithFlags(&done_ev[b], hipEventDisableTiming));
+      SBAMD_HIP(hipEventCreateWithFlags(&copied_ev[b], hipEventDisableTiming));
+    }
+    auto ev_free = [&]() {
+      for (int b = 0; b < 2; ++b)
+      {
+        (void)hipEventDestroy(done_ev[b]);
+        (void)hipEventDestroy(copied_ev[b]);
+      }
+    };
+    // Every band's sets and plans are made before the first launch when
+    // their records fit in a quarter of the free memory (freeing device
+    // memory between launches would wait for the device); otherwise each
+    // band is made just before its launch.
+    struct Band
+    {
+      Set rows, prefix;
+      DeviceBuffer sel_rows, sel_prefix;
+      std::unique_ptr<sbamd_plan> cross, self;
+    };
+    std::vector<std::unique_ptr<Band>> band(static_cast<size_t>(nb));
+    const int rb = record_bytes(rs.tt_f64);
+    uint64_t all_bytes = 0;
+    {
+      // records of the PCFs of original index below r: a running sum
+      std::vector<int64_t> mrec(static_cast<size_t>(n));
+      for (int64_t q = 0; q < n; ++q)
+        mrec[static_cast<size_t>(rs.h_perm[q])] = rs.h_roff[q + 1] - rs.h_roff[q];
+      uint64_t below = 0;
+      int64_t r = 0;
+      for (int64_t k = 0; k < nb; ++k)
+      {
+        const uint64_t pre = below;
+        for (; r < bounds[k + 1]; ++r)
+          below += static_cast<uint64_t>(mrec[static_cast<size_t>(r)]);
+        all_bytes += (below - pre + (cs ? 0 : pre)) * static_cast<uint64_t>(rb);
+      }
+    }
+    size_t free_b = 0, total_b = 0;
+    SBAMD_HIP(hipMemGetInfo(&free_b, &total_b));
+    const bool prebuild = all_bytes <= static_cast<uint64_t>(free_b) / 4;
+    auto build = [&](int64_t k) {
+      const int64_t r0 = bounds[k], r1 = bounds[k + 1];
+      const int64_t base = -static_cast<int64_t>(cum(r0));
+      auto bd = std::make_unique<Band>();
+      std::vector<int64_t> sel, pre;
+      for (int64_t q = 0; q < n; ++q)
+      {
+        const int64_t i = rs.h_perm[q];
+        if (i >= r0 && i < r1)
+          sel.push_back(q);
+        else if (!cs && i < r0)
+          pre.push_back(q);
+      }
+      make_subset(rs, sel, bd->rows, bd->sel_rows, st);
+      if (cs)
+        bd->cross = new_plan(ctx, &bd->rows, cs, op, p, SBAMD_OUT_DENSE, &sh, base);
+      else
+      {
+        if (!pre.empty())
+        {
+          make_subset(rs, pre, bd->prefix, bd->sel_prefix, st);
+          bd->cross = new_plan(ctx, &bd->rows, &bd->prefix, op, p, out_kind, &sh, base);
+        }
+        sbamd_tile_shape sq = sh;
+        sq.cols = 0;
+        bd->self = new_plan(ctx, &bd->rows, nullptr, op, p, out_kind, &sq, base);
+      }
+      band[static_cast<size_t>(k)] = std::move(bd);
+    };
+    // band k - 2 used buffer k & 1: wait for its download and copy it out
+    auto retire = [&](int64_t k) {
+      if (k < 0)
+        return;
+      const int b = static_cast<int>(k & 1);
+      SBAMD_HIP(hipEventSynchronize(copied_ev[b]));
+      const uint64_t e0 = cum(bounds[k]), e1 = cum(bounds[k + 1]);
+      parallel_copy(static_cast<unsigned char*>(out_host) + e0 * eb, hbuf[b], static_cast<size_t>(e1 - e0) * eb);
+      jp->progress += e1 - e0;
+      if (!prebuild)
+        band[static_cast<size_t>(k)].reset();
+    };
+    try
+    {
+      if (prebuild)
+        for (int64_t k = 0; k < nb; ++k)
+          build(k);
+      for (int64_t k = 0; k < nb; ++k)
+      {
+        const int b = static_cast<int>(k & 1);
+        retire(k - 2);
+        if (jp->cancel.load())
+          fail(SBAMD_ERR_CANCELLED, "cancelled");
+        if (!prebuild)
+          build(k);
+        Band& bd = *band[static_cast<size_t>(k)];
+        if (bd.cross)
+          execute_range(*bd.cross, dbuf[b].ptr, out_dtype, nullptr, 0, work_units(*bd.cross), st);
+        if (bd.self)
+          execute_range(*bd.self, dbuf[b].ptr, out_dtype, nullptr, 0, work_units(*bd.self), st);
+        SBAMD_HIP(hipEventRecord(done_ev[b], st));
+        SBAMD_HIP(hipStreamWaitEvent(cst, done_ev[b], 0));
+        SBAMD_HIP(hipMemcpyAsync(hbuf[b], dbuf[b].ptr, static_cast<size_t>(cum(bounds[k + 1]) - cum(bounds[k])) * eb,
+                                 hipMemcpyDeviceToHost, cst));
+        SBAMD_HIP(hipEventRecord(copied_ev[b], cst));
+      }
+      retire(nb - 2);
+      retire(nb - 1);
+      SBAMD_HIP(hipStreamSynchronize(st));
+    }
+    catch (...)
+    {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamSynchronize(cst);
+      ev_free();
+      throw;
+    }
+    ev_free();
+  }
+
+  void run_job(Ctx* ctx, sbamd_job* jp, const sbamd_pcf_set& rows, const sbamd_pcf_set* cols, int32_t op,
+               double p, int32_t out_kind, int32_t precision, void* out_host, int32_t out_dtype)
+  {
+    Set rs, cs;
+    make_set_auto(ctx, rows, precision, rs);
+    if (cols)
+      make_set_auto(ctx, *cols, precision, cs);
+    const size_t eb = out_dtype == SBAMD_F64 ? 8 : 4;
+    const uint64_t nr = static_cast<uint64_t>(rs.n);
+    const uint64_t out_count = cols ? nr * static_cast<uint64_t>(cs.n)
+                                    : (out_kind == SBAMD_OUT_SYMMAT ? nr * (nr + 1) / 2 : (nr > 1 ? nr * (nr - 1) / 2 : 0));
+    const uint64_t out_bytes = out_count * eb;
+    uint64_t budget = ctx->out_budget;
+    if (budget == 0)
+    {
+      size_t free_b = 0, total_b = 0;
+      SBAMD_HIP(hipMemGetInfo(&free_b, &total_b));
+      budget = static_cast<uint64_t>(free_b) / 2;
+    }
+    if (out_bytes <= JOB_SINGLE_MAX_BYTES && out_bytes <= budget)
+    {
+      job_single(ctx, jp, rs, cols ? &cs : nullptr, op, p, out_kind, out_host, out_dtype);
+      return;
+    }
+    // two band buffers within the budget, and at least eight bands for overlap
+    uint64_t chunk = std::min<uint64_t>(budget / (2 * eb), (out_count + 7) / 8);
+    chunk = std::max<uint64_t>(chunk, 1);
+    job_bands(ctx, jp, rs, cols ? &cs : nullptr, op, p, out_kind, out_host, out_dtype, chunk);
+  }
+}
+
 extern "C"
 {
   const char* sbamd_last_error(void) { return g_last_error.c_str(); }
@@ -965,6 +1372,30 @@ extern "C"
       need(ctx, "ctx");
       use_device(ctx);
       SBAMD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+  }
+
+  int sbamd_ctx_set_output_budget(sbamd_ctx* ctx, uint64_t bytes)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      ctx->out_budget = bytes;
+    });
+  }
+
+  int sbamd_ctx_set_tile_shape(sbamd_ctx* ctx, const sbamd_tile_shape* shape)
+  {
+    return guarded([&] {
+      need(ctx, "ctx");
+      need(shape, "shape");
+      const int R = shape->rows ? shape->rows : 16, C = shape->cols ? shape->cols : 16;
+      if (!tile_shape_ok(R, C, shape->cap ? shape->cap : 48, shape->lane_pairs ? shape->lane_pairs : 1))
+        fail(SBAMD_ERR_INVALID, "tile shape out of range (rows + cols <= 64, lane_pairs 1 or 2, "
+                                "at most 512 lanes, 2 <= cap <= 4096)");
+      ctx->job_R = shape->rows;
+      ctx->job_C = shape->cols;
+      ctx->job_cap = shape->cap;
+      ctx->job_ppl = shape->lane_pairs;
     });
   }
 
@@ -1190,74 +1621,9 @@ extern "C"
       need(rows, "rows");
       need(out, "out");
       *out = nullptr;
-      if (op == SBAMD_OP_LP && !(p >= 1.0))
-        fail(SBAMD_ERR_INVALID, "p must be >= 1.");
-      (void)kernel_op(op, p);
-      auto pl = std::make_unique<sbamd_plan>();
-      pl->ctx = ctx;
-      pl->op = op;
-      pl->p = op == SBAMD_OP_LP ? p : 1.0;
-      pl->out_kind = out_kind;
-      pl->precision = rows->precision;
-      pl->tt_f64 = rows->tt_f64;
-      pl->tv_f64 = rows->tv_f64;
-      pl->sets.push_back(rows);
-      if (cols)
-      {
-        if (out_kind != SBAMD_OUT_DENSE)
-          fail(SBAMD_ERR_INVALID, "cross plans (cols != NULL) need SBAMD_OUT_DENSE");
-        if (cols->precision != rows->precision || cols->tt_f64 != rows->tt_f64 ||
-            cols->tv_f64 != rows->tv_f64)
-          fail(SBAMD_ERR_INVALID, "row and column sets must share dtype and precision");
-        pl->cdist = true;
-        pl->sets.push_back(cols);
-        pl->pairs = static_cast<uint64_t>(rows->n) * static_cast<uint64_t>(cols->n);
-        pl->out_count = pl->pairs;
-        pl->segments = static_cast<uint64_t>(cols->n) * rows->records +
-                       static_cast<uint64_t>(rows->n) * cols->records - pl->pairs;
-      }
-      else
-      {
-        const uint64_t n = static_cast<uint64_t>(rows->n);
-        const uint64_t M = static_cast<uint64_t>(rows->records);
-        if (out_kind == SBAMD_OUT_DISTMAT)
-        {
-          pl->pairs = n > 1 ? tri(n) : 0;
-          pl->segments = n > 1 ? (n - 1) * M - pl->pairs : 0;
-        }
-        else if (out_kind == SBAMD_OUT_SYMMAT)
-        {
-          pl->pairs = n * (n + 1) / 2;
-          pl->segments = n > 0 ? n * M - (n > 1 ? tri(n) : 0) : 0;
-        }
-        else
-          fail(SBAMD_ERR_INVALID, "pdist / kernel plans need SBAMD_OUT_DISTMAT or SBAMD_OUT_SYMMAT");
-        pl->out_count = pl->pairs;
-      }
-      Problem pr{0, 0, 0, rows->n};
-      pl->h_probs.push_back(pr);
-      pl->input_bytes = 0;
-      for (const Set* s : pl->sets)
-        pl->input_bytes += s->rec.bytes + s->roff.bytes;
-      if (shape)
-      {
-        if (shape->rows)
-          pl->R = shape->rows;
-        if (shape->cols)
-          pl->C = shape->cols;
-        if (shape->cap)
-          pl->cap = shape->cap;
-        if (shape->lane_pairs)
-          pl->ppl = shape->lane_pairs;
-      }
-      if (!pl->cdist)
-        pl->C = pl->R;
-      check_shape(pl->R, pl->C, pl->cap, pl->ppl);
-      pl->rebuild_tiles();
-      decide_long(*pl);
-      if (!pl->cdist)
-        upload_ooff(*pl);
-      *out = pl.release();
+      if (cols && out_kind != SBAMD_OUT_DENSE)
+        fail(SBAMD_ERR_INVALID, "cross plans (cols != NULL) need SBAMD_OUT_DENSE");
+      *out = new_plan(ctx, rows, cols, op, p, out_kind, shape, 0).release();
     });
   }
 
@@ -1883,52 +2249,8 @@ extern "C"
         {
           SBAMD_HIP(hipSetDevice(ctx->device));
           if (total > 0)
-          {
-            sbamd_set rs, cs;
-            make_set_auto(ctx, rcopy, precision, rs);
-            if (has_cols)
-              make_set_auto(ctx, ccopy, precision, cs);
-            sbamd_plan* pl = nullptr;
-            int rc = sbamd_plan_create(ctx, &rs, has_cols ? &cs : nullptr, op, p, out_kind, nullptr, &pl);
-            if (rc != SBAMD_OK)
-              fail(rc, g_last_error);
-            std::unique_ptr<sbamd_plan> plan(pl);
-            const size_t eb = out_dtype == SBAMD_F64 ? 8 : 4;
-            DeviceBuffer dout(static_cast<size_t>(plan->out_count) * eb);
-            hipStream_t st;
-            SBAMD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-            std::unique_ptr<std::remove_pointer<hipStream_t>::type, decltype(&hipStreamDestroy)> sguard(st, &hipStreamDestroy);
-            const int64_t nt = work_units(*plan);
-            const int64_t chunks = std::min<int64_t>(nt, 64);
-            const bool diag = out_kind == SBAMD_OUT_SYMMAT;
-            hipEvent_t ev;
-            SBAMD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-            std::unique_ptr<std::remove_pointer<hipEvent_t>::type, decltype(&hipEventDestroy)> eguard(ev, &hipEventDestroy);
-            for (int64_t c = 0; c < chunks; ++c)
-            {
-              if (jp->cancel.load())
-                fail(SBAMD_ERR_CANCELLED, "cancelled");
-              const int64_t t0 = nt * c / chunks, t1 = nt * (c + 1) / chunks;
-              execute_range(*plan, dout.ptr, out_dtype, nullptr, t0, t1, st);
-              SBAMD_HIP(hipEventRecord(ev, st));
-              while (hipEventQuery(ev) == hipErrorNotReady)
-              {
-                if (jp->cancel.load())
-                  break;
-                std::this_thread::sleep_for(std::chrono::microseconds(200));
-              }
-              SBAMD_HIP(hipStreamSynchronize(st));
-              uint64_t done = 0;
-              if (plan->use_long)
-                done = static_cast<uint64_t>(t1 - t0);
-              else
-                for (int64_t k = t0; k < t1; ++k)
-                  done += tile_pairs(plan->h_tiles[k], diag);
-              jp->progress += done;
-            }
-            SBAMD_HIP(hipMemcpy(out_host, dout.ptr, static_cast<size_t>(plan->out_count) * eb,
-                                hipMemcpyDeviceToHost));
-          }
+            run_job(ctx, jp, rcopy, has_cols ? &ccopy : nullptr, op, p, out_kind, precision, out_host,
+                    out_dtype);
         }
         catch (const Error& e)
         {

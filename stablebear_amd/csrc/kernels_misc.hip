@@ -197,4 +197,42 @@ namespace sbamd
     hipLaunchKernelGGL(fill_zero_multi_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, L);
     SBAMD_HIP(hipGetLastError());
   }
+
+  // Thread per record of a subset set: the subset's PCF p is PCF sel[p] of
+  // the full set (sorted positions, kept in the full set's order), so record
+  // k of p is record roff[sel[p]] + k of the full set.
+  template <typename W>
+  __global__ void gather_subset_kernel(const int64_t* __restrict__ roff, const W* __restrict__ rec,
+                                       const int64_t* __restrict__ sel, const int64_t* __restrict__ droff,
+                                       int64_t n, int64_t records, W* __restrict__ drec)
+  {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= records)
+      return;
+    int64_t lo = 0, hi = n; // droff[lo] <= r < droff[lo + 1]
+    while (hi - lo > 1)
+    {
+      const int64_t mid = (lo + hi) >> 1;
+      if (droff[mid] <= r)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    drec[r] = rec[roff[sel[lo]] + (r - droff[lo])];
+  }
+
+  void launch_gather_subset(const int64_t* roff, const void* rec, int rec_bytes, const int64_t* sel,
+                            const int64_t* droff, int64_t n, int64_t records, void* drec, hipStream_t stream)
+  {
+    if (records <= 0)
+      return;
+    const unsigned grid = static_cast<unsigned>((records + 255) / 256);
+    if (rec_bytes == 16)
+      hipLaunchKernelGGL(gather_subset_kernel<uint4>, dim3(grid), dim3(256), 0, stream, roff,
+                         static_cast<const uint4*>(rec), sel, droff, n, records, static_cast<uint4*>(drec));
+    else
+      hipLaunchKernelGGL(gather_subset_kernel<uint2>, dim3(grid), dim3(256), 0, stream, roff,
+                         static_cast<const uint2*>(rec), sel, droff, n, records, static_cast<uint2*>(drec));
+    SBAMD_HIP(hipGetLastError());
+  }
 }

@@ -231,6 +231,10 @@ namespace sbamd
     int cu_count = 0;
     size_t lds_per_block = 0;
     void* scratch = nullptr; // this device's address of the no-alloc working memory
+    // host-to-host jobs: output bytes the device may hold at once (0: from
+    // free memory) and the tile shape of their plans (0 fields: defaults)
+    uint64_t out_budget = 0;
+    int32_t job_R = 0, job_C = 0, job_cap = 0, job_ppl = 0;
   };
 
   struct Set
@@ -306,6 +310,8 @@ namespace sbamd
   void launch_norm(const NormParams& prm, int tt_f64, int tv_f64, hipStream_t stream);
   void launch_backward(const BwdParams& prm, hipStream_t stream);
   void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream);
+  void launch_gather_subset(const int64_t* roff, const void* rec, int rec_bytes, const int64_t* sel,
+                            const int64_t* droff, int64_t n, int64_t records, void* drec, hipStream_t stream);
   constexpr int FILL_MAX = 8;
   void launch_fill_zero_multi(void* const* ptrs, const size_t* bytes, int n, hipStream_t stream);
 

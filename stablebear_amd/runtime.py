@@ -190,6 +190,18 @@ def done_task(desc="", total=0) -> Task:
     return Task(None, None, desc, total)
 
 
+def set_output_budget(nbytes: int, device: int | None = None) -> None:
+    """Device bytes a host-to-host job may give its output at once (0: half
+    the free memory); larger outputs run in bands of rows."""
+    L.check(L.lib().sbamd_ctx_set_output_budget(context(device), int(nbytes)))
+
+
+def set_job_tile_shape(rows=0, cols=0, cap=0, lane_pairs=0, device: int | None = None) -> None:
+    """Tile shape of the host-to-host jobs' plans (0: defaults)."""
+    sh = L.TileShape(int(rows), int(cols), int(cap), int(lane_pairs))
+    L.check(L.lib().sbamd_ctx_set_tile_shape(context(device), ctypes.byref(sh)))
+
+
 def submit(rows: Csr, cols: Csr | None, op: int, p: float, out_kind: int, precision: int,
            out: np.ndarray, desc: str, total: int) -> Task:
     """Start a host-to-host pairwise job writing into ``out`` (kept alive by the task)."""

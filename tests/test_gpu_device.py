@@ -222,6 +222,48 @@ def test_job_progress_and_cancel(gpu):
         task2.wait()
 
 
+@pytest.mark.parametrize("kind", ["pdist", "kernel", "cdist"])
+def test_job_bands_under_small_output_budget(gpu, kind):
+    """An output far above the output budget runs in bands of rows (each a
+    cross plan against the earlier rows plus the band's own pairs, downloads
+    overlapped); FP64 results stay bit-exact with the oracle, FP32C within
+    1e-6, and progress reaches the total."""
+    off, t, v = workloads.c2(1500, seed=41)
+    csr = runtime.csr_from_arrays(off, t, v)
+    n = 1500
+    if kind == "cdist":
+        yo, yt, yv = workloads.c2(700, seed=42)
+        cols = runtime.csr_from_arrays(yo, yt, yv)
+        total = n * 700
+        op, okind, out_kind = L.OP_L1, oracle.KIND_CDIST, L.OUT_DENSE
+        exp = oracle.pairwise(oracle.OP_L1, 1.0, okind, off, t, v, yo, yt, yv)
+    elif kind == "kernel":
+        cols, total = None, n * (n + 1) // 2
+        op, okind, out_kind = L.OP_DOT, oracle.KIND_KERNEL, L.OUT_SYMMAT
+        exp = oracle.pairwise(oracle.OP_DOT, 1.0, okind, off, t, v)
+    else:
+        cols, total = None, n * (n - 1) // 2
+        op, okind, out_kind = L.OP_L1, oracle.KIND_PDIST, L.OUT_DISTMAT
+        exp = oracle.pairwise(oracle.OP_L1, 1.0, okind, off, t, v)
+    try:
+        runtime.set_output_budget(total * 8 // 9)  # bands of ~1/18 of the output, two buffers
+        out = np.full(total, np.nan)
+        task = runtime.submit(csr, cols, op, 1.0, out_kind, L.PREC_FP64, out, "d", total)
+        task.wait()
+        assert task.work_completed() == total
+        np.testing.assert_array_equal(out, exp)
+        out32 = np.full(total, np.nan, dtype=np.float32)
+        task = runtime.submit(csr, cols, op, 1.0, out_kind, L.PREC_FP32C, out32, "d", total)
+        task.wait()
+        if kind == "kernel":
+            scale = oracle.pairwise(oracle.OP_DOT, 1.0, okind, off, t, np.abs(v))
+            assert np.max(np.abs(out32 - exp) / scale) <= 1e-6
+        else:
+            assert np.max(np.abs(out32 - exp) / exp) <= 1e-6
+    finally:
+        runtime.set_output_budget(0)
+
+
 def test_device_set_value_update(gpu):
     off, t, v = workloads.c2(80, seed=12)
     ds, pl = _plan(off, t, v)

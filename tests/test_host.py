@@ -265,9 +265,12 @@ def test_system_knobs():
         sb.system.force_cpu(True)
     sb.system.force_cpu(False)
     sb.system.set_cuda_threshold(0)
-    sb.system.set_block_size(16, 16)
     with pytest.raises(ValueError):
-        sb.system.set_block_size(32, 32)
+        sb.system.set_block_size(40, 40)
+    with pytest.raises(ValueError):
+        sb.system.set_tile_cap(1)
+    with pytest.raises(ValueError):
+        sb.system.set_output_budget(-1)
     sb.system.set_precision("fp32c")
     assert sb.system.get_precision() == "fp32c"
     sb.system.set_precision("fp64")
