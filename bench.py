@@ -598,7 +598,7 @@ def _batched_leg(args, stream, cpu_seconds=10.0):
     import torch
 
     import oracle
-    from stablebear_amd import workloads
+    from stablebear_amd import runtime, workloads
     from stablebear_amd.batched import BatchedPlan
 
     t0 = time.perf_counter()
@@ -609,8 +609,29 @@ def _batched_leg(args, stream, cpu_seconds=10.0):
     plan = bp.plan
     setup_s = time.perf_counter() - t0
     out = torch.empty(bp.out_count, dtype=torch.float32, device="cuda")
+    # one-shape baseline (the default tile, every length class in one launch)
+    ms_one, _ = _events_ms(lambda: bp.execute(out.data_ptr(), False, stream.cuda_stream), 3, 1,
+                           stream)
+    t0 = time.perf_counter()
+    h0, m0 = runtime.autotune_cache_stats()
     shape, tune_ms = plan.autotune(out.data_ptr(), False, reps=1)
-    _progress(f"batched: autotuned tile {shape} ({tune_ms:.1f} ms)")
+    tuned_classes = plan.class_shapes()
+    kept_per_class = sum(c[4] > 0 for c in tuned_classes) > 1
+    tune_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plan.autotune(out.data_ptr(), False, reps=1)  # every class from the cache
+    retune_s = time.perf_counter() - t0
+    h1, m1 = runtime.autotune_cache_stats()
+    classes = plan.class_shapes()
+    # the per-class winners in their own launches, whether or not the tuner kept them
+    keep_shape = plan.shape + (plan.lane_pairs,)
+    plan.set_class_shapes([c[:4] for c in classes])
+    ms_per_class, _ = _events_ms(lambda: bp.execute(out.data_ptr(), False, stream.cuda_stream), 3, 1,
+                                 stream)
+    if not kept_per_class:
+        plan.set_shape(*keep_shape)
+    _progress(f"batched: per-class shapes {classes} ({tune_ms:.1f} ms; tuning {tune_s:.2f} s, "
+              f"cached {retune_s:.2f} s)")
     g = bp.graph(out.data_ptr(), False)
     ms_graph, _ = _events_ms(lambda: g.launch(stream.cuda_stream), args.steps, args.warmup, stream)
     ms_eager, _ = _events_ms(lambda: bp.execute(out.data_ptr(), False, stream.cuda_stream),
@@ -643,6 +664,12 @@ def _batched_leg(args, stream, cpu_seconds=10.0):
                     "m ~ round(LogNormal(4, 1)) clipped to [5, 20000], integer values, FP32C",
         "problems": len(probs), "pcfs": int(sum(bp.sizes)), "pairs": plan.pairs,
         "segments": plan.segments, "tile": list(shape),
+        "class_shapes": [{"rows": c[0], "cols": c[1], "cap": c[2], "lane_pairs": c[3],
+                          "tiles_in_own_launch": c[4]} for c in plan.class_shapes()],
+        "one_shape_ms": ms_one, "per_class_ms": ms_per_class,
+        "tuned_choice": "per-class launches" if kept_per_class else "one launch of one shape",
+        "tune_s": tune_s, "cached_retune_s": retune_s,
+        "tune_cache_hits_misses": [h1 - h0, m1 - m0],
         "graph_ms": ms_graph, "eager_ms": ms_eager,
         "pairs_per_s": plan.pairs / (ms_graph * 1e-3),
         "roofline": _roofline(FLOPS_PER_SEGMENT["l1"], plan.segments, ms_graph),

@@ -272,12 +272,36 @@ int sbamd_graph_create(sbamd_plan* plan, void* out_device, int32_t out_dtype, sb
 int sbamd_graph_launch(sbamd_graph* graph, void* stream);
 int sbamd_graph_destroy(sbamd_graph* graph);
 
-/* ---- autotuner: times candidate tile shapes on this plan and keeps the best
- * (also returned). Candidates are limited by the LDS budget. For FP32C
- * plans (or plans set to long-pair mode 1) the long-pair kernel is a
- * candidate too; when it wins the plan uses it (sbamd_plan_info long_pairs). ---- */
+/* ---- autotuner ----
+ * PCFs fall into four length classes by record count (below 128, 512, 2048,
+ * and the rest); the pairs whose longer PCF is of class k form class k's
+ * tiles, run in a launch of their own. sbamd_autotune times candidate tile
+ * shapes (limited by the LDS budget) on each class's launch and keeps the
+ * fastest per class, caching it by device architecture, operation, record
+ * layout, plan kind, class and a coarse histogram of the class (PCF count,
+ * mean record count, tile count): a cache hit skips the timing (the cache
+ * lives for the process, and in the file SBAMD_TUNE_CACHE names, if set).
+ * Separate class launches each end with a tail of their longest tiles: the
+ * plan keeps the per-class shapes only when their launches beat one
+ * longest-first launch of a single shape (its own or a class winner).
+ * For FP32C plans (or plans set to long-pair mode 1) the long-pair kernel is
+ * a candidate for the whole plan too; when it wins the plan uses it
+ * (sbamd_plan_info long_pairs). *best receives the shape of the class with
+ * the most tiles and *best_ms the tuned plan's time. No reference
+ * counterpart beyond the fixed block sizing of cuda_block_scheduler.hpp:29-129. */
 int sbamd_autotune(sbamd_plan* plan, void* out_device, int32_t out_dtype, int32_t reps,
                    sbamd_tile_shape* best, double* best_ms);
+/* The tile shape and launch tile count of each of the four length classes
+ * (shapes[4], tiles[4]; either may be NULL). When every class runs in one
+ * launch its tiles are counted in class 0; the shapes are then the ones the
+ * autotuner found per class (or the plan's, if none were tuned or set). */
+int sbamd_plan_class_shapes(const sbamd_plan* plan, sbamd_tile_shape* shapes, int64_t* tiles);
+/* Give each length class its own tile shape (shapes[4]; zero fields: the
+ * plan's); its tiles then run in a launch of their own. */
+int sbamd_plan_set_class_shapes(sbamd_plan* plan, const sbamd_tile_shape* shapes);
+/* Length classes the autotuner found in its cache (hits) or had to time
+ * (misses), since the process started. */
+int sbamd_autotune_cache_stats(uint64_t* hits, uint64_t* misses);
 int sbamd_plan_set_shape(sbamd_plan* plan, const sbamd_tile_shape* shape);
 
 /* ---- asynchronous host-to-host jobs (the reference's StoppableTask) ----

@@ -136,6 +136,9 @@ _SIGS = {
     "sbamd_autotune": (None, [vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(TileShape),
                               c_dblp]),
     "sbamd_plan_set_shape": (None, [vp, ctypes.POINTER(TileShape)]),
+    "sbamd_plan_class_shapes": (None, [vp, ctypes.POINTER(TileShape), c_i64p]),
+    "sbamd_autotune_cache_stats": (None, [c_u64p, c_u64p]),
+    "sbamd_plan_set_class_shapes": (None, [vp, ctypes.POINTER(TileShape)]),
     "sbamd_pairwise_submit": (None, [vp, ctypes.POINTER(PcfSet), ctypes.POINTER(PcfSet),
                                      ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
                                      ctypes.c_int32, vp, ctypes.c_int32, ctypes.POINTER(vp)]),

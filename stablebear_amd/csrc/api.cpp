@@ -5,10 +5,13 @@
 #include "sbamd_internal.hpp"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -549,7 +552,7 @@ namespace sbamd
       upload(pl.probs, pl.h_probs);
       upload(pl.tiles, pl.h_tiles);
       if (!pl.counter.ptr)
-        pl.counter = DeviceBuffer(16);
+        pl.counter = DeviceBuffer(64 * NCLASS); // one tile counter per class launch, 64 bytes apart
     }
 
     FwdParams fwd_params(const Plan& pl, void* out, int32_t out_dtype, double* row_sum)
@@ -666,19 +669,32 @@ namespace sbamd
                     pl.precision == SBAMD_PREC_FP32C ? 1 : 0, pl.ctx->cu_count, stream);
         return;
       }
-      if (fwd_lds_bytes(pl.tt_f64, pl.tv_f64, pl.R, pl.C, pl.cap) > pl.ctx->lds_per_block)
-        fail(SBAMD_ERR_INVALID, "tile shape needs more LDS than a workgroup may use");
-      FwdParams P = fwd_params(pl, out, out_dtype, row_sum);
-      P.tiles = pl.tiles.as<Tile>() + t0;
-      P.ntiles = t1 - t0;
-      if (pl.persistent && P.ntiles > 0)
-      {
+      if (pl.persistent && t1 > t0)
         // reset by a kernel, not hipMemsetAsync, so graph replays stay correct
         launch_fill_zero(pl.counter.ptr, pl.counter.bytes, stream);
-        P.work_counter = pl.counter.as<uint32_t>();
+      // one launch per length class (a single one without per-class
+      // shapes), the classes of longer PCFs first
+      for (int k = NCLASS - 1; k >= 0; --k)
+      {
+        const int64_t a = std::max(t0, pl.class_begin[k]), b = std::min(t1, pl.class_begin[k + 1]);
+        if (a >= b)
+          continue;
+        const Shape sh = pl.shape_of(k);
+        if (fwd_lds_bytes(pl.tt_f64, pl.tv_f64, sh.R, sh.C, sh.cap) > pl.ctx->lds_per_block)
+          fail(SBAMD_ERR_INVALID, "tile shape needs more LDS than a workgroup may use");
+        FwdParams P = fwd_params(pl, out, out_dtype, row_sum);
+        P.R = sh.R;
+        P.C = sh.C;
+        P.cap = sh.cap;
+        P.ppl = sh.ppl;
+        P.stride = sh.R + sh.C;
+        P.tiles = pl.tiles.as<Tile>() + a;
+        P.ntiles = b - a;
+        if (pl.persistent)
+          P.work_counter = reinterpret_cast<uint32_t*>(pl.counter.as<unsigned char>() + 64 * k);
+        launch_forward(P, pl.tt_f64, pl.tv_f64, kernel_op(pl.op, pl.p),
+                       pl.precision == SBAMD_PREC_FP32C ? 1 : 0, stream);
       }
-      launch_forward(P, pl.tt_f64, pl.tv_f64, kernel_op(pl.op, pl.p),
-                     pl.precision == SBAMD_PREC_FP32C ? 1 : 0, stream);
     }
 
     void check_out_dtype(int32_t d)
@@ -884,6 +900,172 @@ namespace sbamd
       return pl;
     }
 
+    // Autotuner candidates: 32 x 32, 16 x 16 and (plans too small to fill
+    // the device with 16 x 16 tiles) 8 x 8 tiles, one or two pairs per lane,
+    // LDS windows that leave room for two or more workgroups per CU.
+    std::vector<Shape> tune_candidates(const Plan& plan)
+    {
+      std::vector<Shape> cands;
+      const int sides[] = {32, 16, 8};
+      const int caps[] = {32, 48, 64, 96};
+      const size_t budget = std::min<size_t>(plan.ctx->lds_per_block, 80 * 1024);
+      uint64_t mid_tiles = 0; // 16 x 16 tiles of the plan
+      for (size_t b = 0; b < plan.sets.size(); ++b)
+      {
+        const uint64_t nr = (static_cast<uint64_t>(plan.sets[b]->n) + 15) / 16;
+        if (plan.cdist)
+        {
+          mid_tiles = nr * ((static_cast<uint64_t>(plan.sets[1]->n) + 15) / 16);
+          break;
+        }
+        mid_tiles += nr * (nr + 1) / 2;
+      }
+      const bool small = mid_tiles < 4ull * static_cast<uint64_t>(plan.ctx->cu_count);
+      for (int R : sides)
+        for (int C : sides)
+        {
+          if (!plan.cdist && R != C)
+            continue;
+          if ((R < 16 || C < 16) && !small)
+            continue;
+          for (int ppl = 1; ppl <= 2; ++ppl)
+          {
+            if (!tile_shape_ok(R, C, 2, ppl))
+              continue;
+            for (int cap : caps)
+              if (fwd_lds_bytes(plan.tt_f64, plan.tv_f64, R, C, cap) <= budget)
+                cands.push_back(Shape{R, C, cap, ppl});
+          }
+        }
+      return cands;
+    }
+
+    // Cache of tuned shapes, keyed by device architecture, operation,
+    // record layout, plan kind, length class and a coarse histogram of that
+    // class (log2 buckets of its PCF count, mean record count and pair
+    // count). Kept for the process; with SBAMD_TUNE_CACHE naming a file it
+    // is also read from and appended to that file (one "key R C cap ppl"
+    // line per entry).
+    std::mutex g_tune_mu;
+    std::unordered_map<std::string, Shape>& tune_cache()
+    {
+      static std::unordered_map<std::string, Shape> m;
+      static bool loaded = false;
+      if (!loaded)
+      {
+        loaded = true;
+        if (const char* path = std::getenv("SBAMD_TUNE_CACHE"))
+          if (FILE* f = std::fopen(path, "r"))
+          {
+            char key[512];
+            Shape sh;
+            while (std::fscanf(f, "%511s %d %d %d %d", key, &sh.R, &sh.C, &sh.cap, &sh.ppl) == 5)
+              if (tile_shape_ok(sh.R, sh.C, sh.cap, sh.ppl))
+                m[key] = sh;
+            std::fclose(f);
+          }
+      }
+      return m;
+    }
+
+    std::atomic<uint64_t> g_tune_hits{0}, g_tune_misses{0};
+
+    bool tune_cache_get(const std::string& key, Shape& sh)
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto& m = tune_cache();
+      auto it = m.find(key);
+      if (it == m.end())
+      {
+        ++g_tune_misses;
+        return false;
+      }
+      ++g_tune_hits;
+      sh = it->second;
+      return true;
+    }
+
+    void tune_cache_put(const std::string& key, const Shape& sh)
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      tune_cache()[key] = sh;
+      if (const char* path = std::getenv("SBAMD_TUNE_CACHE"))
+        if (FILE* f = std::fopen(path, "a"))
+        {
+          std::fprintf(f, "%s %d %d %d %d\n", key.c_str(), sh.R, sh.C, sh.cap, sh.ppl);
+          std::fclose(f);
+        }
+    }
+
+    int log2_bucket(uint64_t x)
+    {
+      int b = 0;
+      while (x > 1)
+      {
+        x >>= 1;
+        ++b;
+      }
+      return b;
+    }
+
+    std::string tune_key(const Plan& pl, int k)
+    {
+      uint64_t pcfs = 0, recs = 0;
+      for (size_t b = 0; b < (pl.cdist ? 1 : pl.sets.size()); ++b)
+      {
+        const Set& s = *pl.sets[b];
+        for (int64_t q = 0; q < s.n; ++q)
+        {
+          const int64_t m = s.h_roff[q + 1] - s.h_roff[q];
+          if (length_class(m) == k)
+          {
+            ++pcfs;
+            recs += static_cast<uint64_t>(m);
+          }
+        }
+      }
+      // pairs whose longer PCF is of class k (independent of the tile shapes)
+      uint64_t pairs = 0;
+      if (pl.cdist)
+      {
+        uint64_t r_k = 0, r_lt = 0, c_k = 0, c_le = 0;
+        for (int64_t q = 0; q < pl.sets[0]->n; ++q)
+        {
+          const int c = length_class(pl.sets[0]->h_roff[q + 1] - pl.sets[0]->h_roff[q]);
+          r_k += c == k;
+          r_lt += c < k;
+        }
+        for (int64_t q = 0; q < pl.sets[1]->n; ++q)
+        {
+          const int c = length_class(pl.sets[1]->h_roff[q + 1] - pl.sets[1]->h_roff[q]);
+          c_k += c == k;
+          c_le += c <= k;
+        }
+        pairs = r_k * c_le + r_lt * c_k;
+      }
+      else
+        for (const Set* st : pl.sets)
+        {
+          uint64_t below = 0, in = 0;
+          for (int64_t q = 0; q < st->n; ++q)
+          {
+            const int c = length_class(st->h_roff[q + 1] - st->h_roff[q]);
+            below += c < k;
+            in += c == k;
+          }
+          pairs += in * below + in * (in + 1) / 2;
+        }
+      std::string key = pl.ctx->arch;
+      for (char& c : key)
+        if (c == ' ')
+          c = '_';
+      key += "|op" + std::to_string(kernel_op(pl.op, pl.p)) + "|t" + std::to_string(pl.tt_f64) + "v" +
+             std::to_string(pl.tv_f64) + "|" + (pl.cdist ? "cdist" : (pl.batched ? "batched" : "pdist")) +
+             "|k" + std::to_string(k) + "|n" + std::to_string(log2_bucket(pcfs)) + "|m" +
+             std::to_string(log2_bucket(pcfs ? recs / pcfs : 0)) + "|P" + std::to_string(log2_bucket(pairs));
+      return key;
+    }
+
     // Subset of a device set: the PCFs at sorted positions `sel` (ascending,
     // so still sorted by record count), records gathered on the device. For
     // the forward only (no source points).
@@ -927,40 +1109,94 @@ namespace sbamd
   {
     h_tiles.clear();
     const bool diag = out_kind == SBAMD_OUT_SYMMAT;
-    if (cdist)
+    // Longest tiles first over [b0, b1) of h_tiles (LPT order for a launch
+    // whose tiles come from several problems or regions).
+    auto lpt = [&](size_t b0, size_t b1) {
+      std::vector<std::pair<int64_t, size_t>> cost(b1 - b0);
+      for (size_t k = b0; k < b1; ++k)
+      {
+        const Tile& t = h_tiles[k];
+        const Set& rs = *sets[cdist ? 0 : t.prob];
+        const Set& cs = *sets[cdist ? 1 : t.prob];
+        int64_t mr = 0, mc = 0;
+        for (int q = 0; q < t.nr; ++q)
+          mr = std::max(mr, rs.h_roff[t.r0 + q + 1] - rs.h_roff[t.r0 + q]);
+        for (int q = 0; q < t.nc; ++q)
+          mc = std::max(mc, cs.h_roff[t.c0 + q + 1] - cs.h_roff[t.c0 + q]);
+        cost[k - b0] = {-(mr + mc), k};
+      }
+      std::stable_sort(cost.begin(), cost.end());
+      std::vector<Tile> sorted(b1 - b0);
+      for (size_t k = 0; k < cost.size(); ++k)
+        sorted[k] = h_tiles[cost[k].second];
+      std::copy(sorted.begin(), sorted.end(), h_tiles.begin() + static_cast<int64_t>(b0));
+    };
+    // sorted positions where each length class starts
+    auto class_bounds = [](const std::vector<int64_t>& m, int64_t* at) {
+      for (int k = 0; k <= NCLASS; ++k)
+        at[k] = static_cast<int64_t>(std::lower_bound(m.begin(), m.end(), CLASS_MIN[k]) - m.begin());
+      at[NCLASS] = static_cast<int64_t>(m.size());
+    };
+    if (!per_class)
     {
-      auto mr = sorted_counts(*sets[0]);
-      auto mc = sorted_counts(*sets[1]);
-      build_tiles(mr, &mc, R, C, false, 0, fold_threshold, h_tiles);
+      if (cdist)
+      {
+        auto mr = sorted_counts(*sets[0]);
+        auto mc = sorted_counts(*sets[1]);
+        build_tiles(mr, &mc, R, C, false, 0, fold_threshold, h_tiles);
+      }
+      else
+        for (size_t b = 0; b < sets.size(); ++b)
+        {
+          auto m = sorted_counts(*sets[b]);
+          build_tiles(m, nullptr, R, C, diag, static_cast<uint16_t>(b), fold_threshold, h_tiles);
+        }
+      if (batched)
+        lpt(0, h_tiles.size());
+      class_begin[0] = 0;
+      for (int k = 1; k <= NCLASS; ++k)
+        class_begin[k] = static_cast<int64_t>(h_tiles.size());
     }
     else
     {
-      for (size_t b = 0; b < sets.size(); ++b)
+      // Class k's pairs are those whose longer PCF is of class k: rows of
+      // class k against every shorter-or-equal column, and (cdist) rows of
+      // shorter classes against columns of class k.
+      std::vector<std::vector<int64_t>> ms;
+      std::vector<std::array<int64_t, NCLASS + 1>> at;
+      for (const Set* st : sets)
       {
-        auto m = sorted_counts(*sets[b]);
-        build_tiles(m, nullptr, R, C, diag, static_cast<uint16_t>(b), fold_threshold, h_tiles);
+        ms.push_back(sorted_counts(*st));
+        std::array<int64_t, NCLASS + 1> a{};
+        class_bounds(ms.back(), a.data());
+        at.push_back(a);
       }
-    }
-    if (batched)
-    {
-      // Longest tiles first across all problems (LPT order for the launch).
-      std::vector<std::pair<int64_t, size_t>> cost(h_tiles.size());
-      for (size_t k = 0; k < h_tiles.size(); ++k)
+      class_begin[0] = 0;
+      for (int k = 0; k < NCLASS; ++k)
       {
-        const Tile& t = h_tiles[k];
-        const Set& s = *sets[t.prob];
-        int64_t mr = 0, mc = 0;
-        for (int q = 0; q < t.nr; ++q)
-          mr = std::max(mr, s.h_roff[t.r0 + q + 1] - s.h_roff[t.r0 + q]);
-        for (int q = 0; q < t.nc; ++q)
-          mc = std::max(mc, s.h_roff[t.c0 + q + 1] - s.h_roff[t.c0 + q]);
-        cost[k] = {-(mr + mc), k};
+        const Shape sh = cls[k];
+        const size_t b0 = h_tiles.size();
+        if (cdist)
+        {
+          const auto &ar = at[0], &ac = at[1];
+          build_tiles_region(ms[0], ms[1], ar[k], ar[k + 1], 0, ac[k + 1], sh.R, sh.C, false, false, 0,
+                             fold_threshold, h_tiles);
+          build_tiles_region(ms[0], ms[1], 0, ar[k], ac[k], ac[k + 1], sh.R, sh.C, false, false, 0,
+                             fold_threshold, h_tiles);
+        }
+        else
+          for (size_t b = 0; b < sets.size(); ++b)
+          {
+            const auto& a = at[b];
+            const uint16_t prob = static_cast<uint16_t>(b);
+            build_tiles_region(ms[b], ms[b], a[k], a[k + 1], 0, a[k], sh.R, sh.R, false, diag, prob,
+                               fold_threshold, h_tiles);
+            build_tiles_region(ms[b], ms[b], a[k], a[k + 1], a[k], a[k + 1], sh.R, sh.R, true, diag, prob,
+                               fold_threshold, h_tiles);
+          }
+        lpt(b0, h_tiles.size());
+        class_begin[k + 1] = static_cast<int64_t>(h_tiles.size());
       }
-      std::stable_sort(cost.begin(), cost.end());
-      std::vector<Tile> sorted(h_tiles.size());
-      for (size_t k = 0; k < cost.size(); ++k)
-        sorted[k] = h_tiles[cost[k].second];
-      h_tiles.swap(sorted);
     }
     finish_plan(*this);
     bump();
@@ -1333,6 +1569,7 @@ extern "C"
       SBAMD_HIP(hipGetDeviceProperties(&prop, device));
       ctx->cu_count = prop.multiProcessorCount;
       ctx->lds_per_block = prop.sharedMemPerBlock;
+      ctx->arch = prop.gcnArchName;
       {
         std::lock_guard<std::mutex> lk(g_scratch_mu);
         auto it = scratch_addr().find(device);
@@ -1740,12 +1977,18 @@ extern "C"
       info->out_count = plan->out_count;
       info->tiles = plan->h_tiles.size();
       info->input_bytes = plan->input_bytes;
-      info->rows = plan->R;
-      info->cols = plan->C;
-      info->cap = plan->cap;
+      // with per-class shapes: the class with the most tiles
+      int kb = 0;
+      for (int k = 1; k < NCLASS; ++k)
+        if (plan->class_begin[k + 1] - plan->class_begin[k] > plan->class_begin[kb + 1] - plan->class_begin[kb])
+          kb = k;
+      const Shape sh = plan->shape_of(kb);
+      info->rows = sh.R;
+      info->cols = sh.C;
+      info->cap = sh.cap;
       info->problems = static_cast<int32_t>(plan->h_probs.size());
       info->long_pairs = plan->use_long ? 1 : 0;
-      info->lane_pairs = plan->ppl;
+      info->lane_pairs = sh.ppl;
     });
   }
 
@@ -1809,6 +2052,7 @@ extern "C"
       plan->C = C;
       plan->cap = cap;
       plan->ppl = ppl;
+      plan->per_class = false;
       plan->rebuild_tiles();
     });
   }
@@ -2075,138 +2319,206 @@ extern "C"
       hipStream_t st = plan->ctx->stream;
       if (plan->batched)
         refresh_batch(*plan, st);
-      const int R0 = plan->R, C0 = plan->C, cap0 = plan->cap, ppl0 = plan->ppl;
       plan->use_long = false; // tile candidates first
-      struct Cand
-      {
-        int R, C, cap, ppl;
-      };
-      std::vector<Cand> cands;
-      // 32 x 32 and 16 x 16 tiles with LDS windows that leave room for two
-      // or more resident workgroups per CU; 8 x 8 only where a plan is too
-      // small to fill the device with 16 x 16 tiles.
-      const int sides[] = {32, 16, 8};
-      const int caps[] = {32, 48, 64, 96};
-      const size_t budget = std::min<size_t>(plan->ctx->lds_per_block, 80 * 1024);
-      uint64_t mid_tiles = 0; // 16 x 16 tiles of the plan
-      for (size_t b = 0; b < plan->sets.size(); ++b)
-      {
-        const uint64_t nr = (static_cast<uint64_t>(plan->sets[b]->n) + 15) / 16;
-        if (plan->cdist)
-        {
-          mid_tiles = nr * ((static_cast<uint64_t>(plan->sets[1]->n) + 15) / 16);
-          break;
-        }
-        mid_tiles += nr * (nr + 1) / 2;
-      }
-      const bool small = mid_tiles < 4ull * static_cast<uint64_t>(plan->ctx->cu_count);
-      for (int R : sides)
-        for (int C : sides)
-        {
-          if (!plan->cdist && R != C)
-            continue;
-          if ((R < 16 || C < 16) && !small)
-            continue;
-          for (int ppl = 1; ppl <= 2; ++ppl)
-          {
-            if (!tile_shape_ok(R, C, 2, ppl))
-              continue;
-            for (int cap : caps)
-            {
-              if (fwd_lds_bytes(plan->tt_f64, plan->tv_f64, R, C, cap) > budget)
-                continue;
-              cands.push_back({R, C, cap, ppl});
-            }
-          }
-        }
+      const std::vector<Shape> cands = tune_candidates(*plan);
       hipEvent_t e0, e1;
       SBAMD_HIP(hipEventCreate(&e0));
       SBAMD_HIP(hipEventCreate(&e1));
-      double bestv = std::numeric_limits<double>::infinity();
-      Cand bc{R0, C0, cap0, ppl0};
+      auto egd = [&]() {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+      };
+      // ms of tile range [t0, t1) (one warm-up launch, then reps)
+      auto time_range = [&](int64_t t0, int64_t t1) {
+        execute_range(*plan, out_device, out_dtype, nullptr, t0, t1, st);
+        SBAMD_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r)
+          execute_range(*plan, out_device, out_dtype, nullptr, t0, t1, st);
+        SBAMD_HIP(hipEventRecord(e1, st));
+        SBAMD_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SBAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return static_cast<double>(ms) / reps;
+      };
+      double tiled_ms = 0.0;
+      bool chose_long = false;
       try
       {
-        for (const Cand& c : cands)
+        // one shape per length class, each timed on its own class's launch
+        if (!plan->per_class)
         {
-          plan->R = c.R;
-          plan->C = c.C;
-          plan->cap = c.cap;
-          plan->ppl = c.ppl;
-          plan->rebuild_tiles();
-          execute_range(*plan, out_device, out_dtype, nullptr, 0,
-                        work_units(*plan), st); // warm-up
-          SBAMD_HIP(hipEventRecord(e0, st));
-          for (int r = 0; r < reps; ++r)
-            execute_range(*plan, out_device, out_dtype, nullptr, 0,
-                          work_units(*plan), st);
-          SBAMD_HIP(hipEventRecord(e1, st));
-          SBAMD_HIP(hipEventSynchronize(e1));
-          float ms = 0.f;
-          SBAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
-          const double per = static_cast<double>(ms) / reps;
-          if (per < bestv)
+          plan->per_class = true;
+          for (int k = 0; k < NCLASS; ++k)
+            plan->cls[k] = Shape{plan->R, plan->C, plan->cap, plan->ppl};
+        }
+        plan->rebuild_tiles();
+        for (int k = 0; k < NCLASS; ++k)
+        {
+          if (plan->class_begin[k + 1] == plan->class_begin[k])
+            continue;
+          const std::string key = tune_key(*plan, k);
+          Shape hit;
+          if (tune_cache_get(key, hit))
           {
-            bestv = per;
-            bc = c;
+            plan->cls[k] = hit;
+            plan->rebuild_tiles();
+            continue;
           }
+          double bestv = std::numeric_limits<double>::infinity();
+          Shape bsh = plan->cls[k];
+          for (const Shape& c : cands)
+          {
+            plan->cls[k] = c;
+            plan->rebuild_tiles();
+            const double ms = time_range(plan->class_begin[k], plan->class_begin[k + 1]);
+            if (ms < bestv)
+            {
+              bestv = ms;
+              bsh = c;
+            }
+          }
+          plan->cls[k] = bsh;
+          plan->rebuild_tiles();
+          tune_cache_put(key, bsh);
+        }
+        tiled_ms = time_range(0, work_units(*plan));
+        // Separate class launches each end with a tail of their longest
+        // tiles, which one longest-first launch fills with shorter ones: one
+        // shape for every class (the plan's own, or a class winner) in one
+        // launch stays when it is faster.
+        {
+          const Shape tuned[NCLASS] = {plan->cls[0], plan->cls[1], plan->cls[2], plan->cls[3]};
+          std::vector<Shape> singles{Shape{plan->R, plan->C, plan->cap, plan->ppl}};
+          for (int k = 0; k < NCLASS; ++k)
+          {
+            bool seen = false;
+            for (const Shape& x : singles)
+              seen |= x.R == tuned[k].R && x.C == tuned[k].C && x.cap == tuned[k].cap && x.ppl == tuned[k].ppl;
+            if (!seen && plan->class_begin[k + 1] > plan->class_begin[k])
+              singles.push_back(tuned[k]);
+          }
+          Shape best_single{};
+          double best_single_ms = std::numeric_limits<double>::infinity();
+          plan->per_class = false;
+          for (const Shape& x : singles)
+          {
+            plan->R = x.R;
+            plan->C = x.C;
+            plan->cap = x.cap;
+            plan->ppl = x.ppl;
+            plan->rebuild_tiles();
+            const double ms = time_range(0, work_units(*plan));
+            if (ms < best_single_ms)
+            {
+              best_single_ms = ms;
+              best_single = x;
+            }
+          }
+          plan->R = best_single.R;
+          plan->C = best_single.C;
+          plan->cap = best_single.cap;
+          plan->ppl = best_single.ppl;
+          if (best_single_ms <= tiled_ms)
+          {
+            tiled_ms = best_single_ms;
+            plan->rebuild_tiles();
+          }
+          else
+          {
+            plan->per_class = true;
+            plan->rebuild_tiles();
+          }
+        }
+        // the long-pair kernel is a candidate too (FP32C plans, or plans
+        // that asked for it)
+        if (!plan->batched && plan->pairs > 0 &&
+            (plan->precision == SBAMD_PREC_FP32C || plan->long_mode == 1))
+        {
+          plan->use_long = true;
+          const double lms = time_range(0, work_units(*plan));
+          chose_long = lms < tiled_ms;
+          if (chose_long)
+            tiled_ms = lms;
         }
       }
       catch (...)
       {
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        egd();
         throw;
       }
-      plan->R = bc.R;
-      plan->C = bc.C;
-      plan->cap = bc.cap;
-      plan->ppl = bc.ppl;
-      plan->rebuild_tiles();
-      // the long-pair kernel is a candidate too (FP32C plans, or plans that
-      // asked for it)
-      bool chose_long = false;
-      if (!plan->batched && plan->pairs > 0 &&
-          (plan->precision == SBAMD_PREC_FP32C || plan->long_mode == 1))
-      {
-        plan->use_long = true;
-        try
-        {
-          execute_range(*plan, out_device, out_dtype, nullptr, 0, work_units(*plan), st);
-          SBAMD_HIP(hipEventRecord(e0, st));
-          for (int r = 0; r < reps; ++r)
-            execute_range(*plan, out_device, out_dtype, nullptr, 0, work_units(*plan), st);
-          SBAMD_HIP(hipEventRecord(e1, st));
-          SBAMD_HIP(hipEventSynchronize(e1));
-        }
-        catch (...)
-        {
-          (void)hipEventDestroy(e0);
-          (void)hipEventDestroy(e1);
-          throw;
-        }
-        float ms = 0.f;
-        SBAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
-        const double per = static_cast<double>(ms) / reps;
-        if (per < bestv)
-        {
-          bestv = per;
-          chose_long = true;
-        }
-      }
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
+      egd();
       plan->use_long = chose_long;
       plan->long_mode = chose_long ? 1 : 0;
       plan->bump();
       if (best)
       {
-        best->rows = bc.R;
-        best->cols = bc.C;
-        best->cap = bc.cap;
-        best->lane_pairs = bc.ppl;
+        // the shape of the class with the most tiles
+        int kb = 0;
+        for (int k = 1; k < NCLASS; ++k)
+          if (plan->class_begin[k + 1] - plan->class_begin[k] > plan->class_begin[kb + 1] - plan->class_begin[kb])
+            kb = k;
+        const Shape sh = plan->shape_of(kb);
+        best->rows = sh.R;
+        best->cols = sh.C;
+        best->cap = sh.cap;
+        best->lane_pairs = sh.ppl;
       }
       if (best_ms)
-        *best_ms = bestv;
+        *best_ms = tiled_ms;
+    });
+  }
+
+  int sbamd_plan_set_class_shapes(sbamd_plan* plan, const sbamd_tile_shape* shapes)
+  {
+    return guarded([&] {
+      need(plan, "plan");
+      need(shapes, "shapes");
+      Shape sh[NCLASS];
+      for (int k = 0; k < NCLASS; ++k)
+      {
+        sh[k] = Shape{shapes[k].rows ? shapes[k].rows : plan->R, shapes[k].cols ? shapes[k].cols : plan->C,
+                      shapes[k].cap ? shapes[k].cap : plan->cap,
+                      shapes[k].lane_pairs ? shapes[k].lane_pairs : plan->ppl};
+        if (!plan->cdist)
+          sh[k].C = sh[k].R;
+        check_shape(sh[k].R, sh[k].C, sh[k].cap, sh[k].ppl);
+      }
+      for (int k = 0; k < NCLASS; ++k)
+        plan->cls[k] = sh[k];
+      plan->per_class = true;
+      plan->rebuild_tiles();
+    });
+  }
+
+  int sbamd_autotune_cache_stats(uint64_t* hits, uint64_t* misses)
+  {
+    return guarded([&] {
+      if (hits)
+        *hits = g_tune_hits.load();
+      if (misses)
+        *misses = g_tune_misses.load();
+    });
+  }
+
+  int sbamd_plan_class_shapes(const sbamd_plan* plan, sbamd_tile_shape* shapes, int64_t* tiles)
+  {
+    return guarded([&] {
+      need(plan, "plan");
+      for (int k = 0; k < NCLASS; ++k)
+      {
+        // the class's own shape once one was set or tuned (kept even when
+        // the plan runs every class in one launch), else the plan's
+        const Shape sh = plan->per_class || plan->cls[k].R ? plan->cls[k] : plan->shape_of(k);
+        if (shapes)
+        {
+          shapes[k].rows = sh.R;
+          shapes[k].cols = sh.C;
+          shapes[k].cap = sh.cap;
+          shapes[k].lane_pairs = sh.ppl;
+        }
+        if (tiles)
+          tiles[k] = plan->class_begin[k + 1] - plan->class_begin[k];
+      }
     });
   }
 

@@ -213,4 +213,52 @@ namespace sbamd
       }
     }
   }
+
+  // Tiles of R x C pairs over rows [r_lo, r_hi) x cols [c_lo, c_hi) of sorted
+  // index space (absolute indices). With `tri` the region is the square
+  // r_lo == c_lo, r_hi == c_hi of a symmetric plan and only its lower
+  // triangle is tiled (square tiles, diagonal tiles flagged). Longest tiles
+  // first, as build_tiles.
+  void build_tiles_region(const std::vector<int64_t>& m_rows, const std::vector<int64_t>& m_cols,
+                          int64_t r_lo, int64_t r_hi, int64_t c_lo, int64_t c_hi, int R, int C, bool tri,
+                          bool include_diag, uint16_t prob, int64_t fold_threshold, std::vector<Tile>& tiles)
+  {
+    if (r_hi <= r_lo || c_hi <= c_lo)
+      return;
+    if (tri && R != C)
+      fail(SBAMD_ERR_INVALID, "symmetric plans need square tiles");
+    const int64_t nbr = (r_hi - r_lo + R - 1) / R;
+    const int64_t nbc = (c_hi - c_lo + C - 1) / C;
+    auto block_max = [](const std::vector<int64_t>& m, int64_t b0, int64_t cnt) {
+      int64_t mx = 0;
+      for (int64_t k = b0; k < b0 + cnt; ++k)
+        mx = std::max(mx, m[k]);
+      return mx;
+    };
+    for (int64_t bi = nbr - 1; bi >= 0; --bi)
+    {
+      const int64_t r0 = r_lo + bi * R;
+      const int64_t rn = std::min<int64_t>(R, r_hi - r0);
+      const int64_t rmax = block_max(m_rows, r0, rn);
+      const int64_t bj_end = tri ? bi : nbc - 1;
+      for (int64_t bj = bj_end; bj >= 0; --bj)
+      {
+        const int64_t c0 = c_lo + bj * C;
+        const int64_t cn = std::min<int64_t>(C, c_hi - c0);
+        const bool diag = tri && bi == bj;
+        if (diag && !include_diag && rn < 2)
+          continue;
+        Tile t;
+        t.r0 = static_cast<int32_t>(r0);
+        t.c0 = static_cast<int32_t>(c0);
+        t.nr = static_cast<int16_t>(rn);
+        t.nc = static_cast<int16_t>(cn);
+        t.prob = prob;
+        t.flags = diag ? TILE_DIAG : 0;
+        if (rmax + block_max(m_cols, c0, cn) - 1 >= fold_threshold)
+          t.flags |= TILE_FOLD;
+        tiles.push_back(t);
+      }
+    }
+  }
 }

@@ -231,6 +231,7 @@ namespace sbamd
     int cu_count = 0;
     size_t lds_per_block = 0;
     void* scratch = nullptr; // this device's address of the no-alloc working memory
+    std::string arch;        // gcnArchName (autotuner cache key)
     // host-to-host jobs: output bytes the device may hold at once (0: from
     // free memory) and the tile shape of their plans (0 fields: defaults)
     uint64_t out_budget = 0;
@@ -256,6 +257,18 @@ namespace sbamd
     DeviceBuffer roff, rec, perm, src;
   };
 
+  // Length classes of the per-class tile shapes: a PCF's class is set by
+  // its record count (thresholds 128, 512, 2048). Sets are sorted by record
+  // count, so a class is a range of sorted positions.
+  constexpr int NCLASS = 4;
+  inline int length_class(int64_t m) { return m < 128 ? 0 : (m < 512 ? 1 : (m < 2048 ? 2 : 3)); }
+  constexpr int64_t CLASS_MIN[NCLASS + 1] = {0, 128, 512, 2048, INT64_MAX};
+
+  struct Shape
+  {
+    int32_t R, C, cap, ppl;
+  };
+
   struct Plan
   {
     Ctx* ctx = nullptr;
@@ -271,6 +284,13 @@ namespace sbamd
     // may pick another)
     int32_t R = 16, C = 16, cap = 48;
     int32_t ppl = 1; // pairs per lane of the tile kernel
+    // per length class shapes (set by the autotuner): the tiles of class k
+    // are h_tiles[class_begin[k], class_begin[k + 1]) and run in a launch of
+    // their own; without them every tile has the shape above, in one launch
+    bool per_class = false;
+    Shape cls[NCLASS] = {};
+    int64_t class_begin[NCLASS + 1] = {};
+    Shape shape_of(int k) const { return per_class ? cls[k] : Shape{R, C, cap, ppl}; }
     int64_t fold_threshold = 8192;
     uint64_t pairs = 0, segments = 0, out_count = 0;
     std::vector<Problem> h_probs;
@@ -377,6 +397,9 @@ namespace sbamd
                     double p, int out_kind, double* out, hipStream_t stream);
 
   // Planning
+  void build_tiles_region(const std::vector<int64_t>& m_rows, const std::vector<int64_t>& m_cols,
+                          int64_t r_lo, int64_t r_hi, int64_t c_lo, int64_t c_hi, int R, int C, bool tri,
+                          bool include_diag, uint16_t prob, int64_t fold_threshold, std::vector<Tile>& tiles);
   void build_tiles(const std::vector<int64_t>& m_rows_sorted, const std::vector<int64_t>* m_cols_sorted,
                    int R, int C, bool include_diag, uint16_t prob, int64_t fold_threshold,
                    std::vector<Tile>& tiles);

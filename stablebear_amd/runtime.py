@@ -202,6 +202,13 @@ def set_job_tile_shape(rows=0, cols=0, cap=0, lane_pairs=0, device: int | None =
     L.check(L.lib().sbamd_ctx_set_tile_shape(context(device), ctypes.byref(sh)))
 
 
+def autotune_cache_stats():
+    """(hits, misses) of the autotuner's per-class shape cache in this process."""
+    h, m = ctypes.c_uint64(), ctypes.c_uint64()
+    L.check(L.lib().sbamd_autotune_cache_stats(ctypes.byref(h), ctypes.byref(m)))
+    return h.value, m.value
+
+
 def submit(rows: Csr, cols: Csr | None, op: int, p: float, out_kind: int, precision: int,
            out: np.ndarray, desc: str, total: int) -> Task:
     """Start a host-to-host pairwise job writing into ``out`` (kept alive by the task)."""
@@ -486,6 +493,19 @@ class Plan:
                                        int(reps), ctypes.byref(sh), ctypes.byref(ms)))
         self.refresh_stats()
         return (sh.rows, sh.cols, sh.cap), ms.value
+
+    def set_class_shapes(self, shapes):
+        """One (rows, cols, cap, lane_pairs) per length class (4 of them)."""
+        arr = (L.TileShape * 4)(*[L.TileShape(*sh) for sh in shapes])
+        L.check(L.lib().sbamd_plan_set_class_shapes(self.handle, arr))
+        self.refresh_stats()
+
+    def class_shapes(self):
+        """[(rows, cols, cap, lane_pairs, tiles)] of the four length classes."""
+        sh = (L.TileShape * 4)()
+        tiles = np.zeros(4, dtype=np.int64)
+        L.check(L.lib().sbamd_plan_class_shapes(self.handle, sh, tiles.ctypes.data_as(L.c_i64p)))
+        return [(s.rows, s.cols, s.cap, s.lane_pairs, int(t)) for s, t in zip(sh, tiles)]
 
     def graph(self, out_ptr: int, out_f64: bool) -> "Graph":
         return Graph(self, out_ptr, out_f64)

@@ -71,6 +71,43 @@ def test_autotune_picks_a_valid_shape(gpu):
     assert np.max(np.abs(got - exp) / exp) <= 1e-6
 
 
+def test_per_class_autotune_bit_exact_and_cached(gpu):
+    """PCFs of several length classes (5..3000 records): the autotuner picks a
+    shape per class, each class's tiles run in a launch of their own, FP64
+    results stay bit-exact, and a second plan of the same shape of data takes
+    every class's shape from the cache."""
+    rng = np.random.default_rng(51)
+    m = np.clip(np.round(rng.lognormal(4.5, 1.2, 600)), 5, 3000).astype(np.int64)
+    off, t = workloads._times(rng, m, 1.0)
+    v = rng.integers(-50, 51, size=int(off[-1])).astype(np.float64)
+    v[off[1:] - 1] = 0.0
+    exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_PDIST, off, t, v)
+    ds, pl = _plan(off, t, v)
+    # every class its own shape and launch
+    pl.set_class_shapes([(32, 32, 64, 2), (16, 16, 48, 1), (16, 16, 96, 1), (8, 8, 32, 2)])
+    cls = pl.class_shapes()
+    assert [c[:4] for c in cls] == [(32, 32, 64, 2), (16, 16, 48, 1), (16, 16, 96, 1), (8, 8, 32, 2)]
+    used = [c for c in cls if c[4] > 0]
+    assert len(used) >= 3 and sum(c[4] for c in cls) == pl.tiles
+    np.testing.assert_array_equal(_run(pl, True), exp)
+    # the autotuner times shapes per class (or keeps one launch of one shape
+    # when that is faster); a second plan of the same data hits the cache
+    out = torch.empty(pl.out_count, dtype=torch.float64, device="cuda")
+    h0, m0 = runtime.autotune_cache_stats()
+    pl.autotune(out.data_ptr(), True, reps=1)
+    h1, m1 = runtime.autotune_cache_stats()
+    assert m1 - m0 == len(used) and h1 == h0
+    np.testing.assert_array_equal(_run(pl, True), exp)
+    ds2, pl2 = _plan(off, t, v)
+    pl2.autotune(out.data_ptr(), True, reps=1)
+    h2, m2 = runtime.autotune_cache_stats()
+    assert h2 - h1 == len(used) and m2 == m1
+    np.testing.assert_array_equal(_run(pl2, True), exp)
+    pl2.set_shape(16, 16, 48, 1)  # back to one shape for every class
+    assert all(c[4] == 0 for c in pl2.class_shapes()[1:])
+    np.testing.assert_array_equal(_run(pl2, True), exp)
+
+
 def test_graph_replays_match_eager(gpu):
     off, t, v = workloads.c2(250, seed=8)
     ds, pl = _plan(off, t, v, prec="fp32c")
