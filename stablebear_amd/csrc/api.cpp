@@ -1153,6 +1153,8 @@ namespace sbamd
         }
       if (batched)
         lpt(0, h_tiles.size());
+      else
+        xcd_supertile_order(h_tiles, 0, h_tiles.size(), R, C, DEVICE_XCDS);
       class_begin[0] = 0;
       for (int k = 1; k <= NCLASS; ++k)
         class_begin[k] = static_cast<int64_t>(h_tiles.size());
@@ -1195,6 +1197,8 @@ namespace sbamd
                                fold_threshold, h_tiles);
           }
         lpt(b0, h_tiles.size());
+        if (!batched)
+          xcd_supertile_order(h_tiles, b0, h_tiles.size(), sh.R, sh.C, DEVICE_XCDS);
         class_begin[k + 1] = static_cast<int64_t>(h_tiles.size());
       }
     }

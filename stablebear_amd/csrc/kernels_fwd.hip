@@ -272,10 +272,12 @@ namespace sbamd
         else
           idx = i * P.out_cols + j;
         idx += pr.out_base;
+        // the outputs are scattered through the permutation: streaming
+        // stores keep them from taking L2 lines that the staged records need
         if (P.out_f64)
-          static_cast<double*>(P.out)[idx] = res;
+          __builtin_nontemporal_store(res, static_cast<double*>(P.out) + idx);
         else
-          static_cast<float*>(P.out)[idx] = static_cast<float>(res);
+          __builtin_nontemporal_store(static_cast<float>(res), static_cast<float*>(P.out) + idx);
         if (P.row_sum)
           atomicAdd(&s_rows[lr], res);
       }

@@ -10,6 +10,7 @@
 #include <limits>
 #include <numeric>
 #include <sstream>
+#include <unordered_map>
 
 namespace sbamd
 {
@@ -260,5 +261,57 @@ namespace sbamd
         tiles.push_back(t);
       }
     }
+  }
+
+  // Reorders tiles [b0, b1) of a launch with one workgroup per tile so that
+  // each XCD works through supertiles, blocks of G x G tiles that share their
+  // R * G rows and C * G columns, and re-reads those PCFs from its own L2.
+  // The dispatcher hands workgroup w to XCD w mod nxcd, so XCD x runs the
+  // tiles at positions congruent to x: the supertiles, taken in the order of
+  // their first tile (the longest first, as the list was), are dealt to the
+  // XCDs in turn, and the XCDs' sequences are interleaved. Consecutive
+  // supertiles of one XCD mostly lie in the same block row, so the rows stay
+  // in its L2 as well. G is the largest of 16, 8 and 4 that still gives every
+  // XCD four supertiles (with fewer, one XCD would hold a large share of the
+  // longest tiles); a range too small for G = 4 keeps its order. Returns G
+  // (0: order kept).
+  int xcd_supertile_order(std::vector<Tile>& tiles, size_t b0, size_t b1, int R, int C, int nxcd)
+  {
+    if (nxcd <= 1 || b1 <= b0 + 1)
+      return 0;
+    auto key = [&](const Tile& t, int G) {
+      return (static_cast<uint64_t>(t.prob) << 48) | (static_cast<uint64_t>(t.r0 / (G * R)) << 24) |
+             static_cast<uint64_t>(t.c0 / (G * C));
+    };
+    for (int G : {16, 8, 4})
+    {
+      std::unordered_map<uint64_t, size_t> id;
+      std::vector<std::vector<Tile>> groups;
+      for (size_t k = b0; k < b1; ++k)
+      {
+        auto it = id.find(key(tiles[k], G));
+        if (it == id.end())
+        {
+          it = id.emplace(key(tiles[k], G), groups.size()).first;
+          groups.emplace_back();
+        }
+        groups[it->second].push_back(tiles[k]);
+      }
+      if (groups.size() < 4 * static_cast<size_t>(nxcd))
+        continue;
+      std::vector<std::vector<Tile>> seq(static_cast<size_t>(nxcd));
+      for (size_t s = 0; s < groups.size(); ++s)
+      {
+        auto& q = seq[s % static_cast<size_t>(nxcd)];
+        q.insert(q.end(), groups[s].begin(), groups[s].end());
+      }
+      size_t pos = b0;
+      for (size_t k = 0; pos < b1; ++k)
+        for (auto& q : seq)
+          if (k < q.size())
+            tiles[pos++] = q[k];
+      return G;
+    }
+    return 0;
   }
 }

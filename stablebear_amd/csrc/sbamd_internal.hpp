@@ -404,4 +404,8 @@ namespace sbamd
                    int R, int C, bool include_diag, uint16_t prob, int64_t fold_threshold,
                    std::vector<Tile>& tiles);
   void sort_by_records(const std::vector<int64_t>& m, std::vector<int64_t>& perm);
+  int xcd_supertile_order(std::vector<Tile>& tiles, size_t b0, size_t b1, int R, int C, int nxcd);
+
+  // XCDs of the device (gfx950: 8, each with its own 4 MB L2).
+  constexpr int DEVICE_XCDS = 8;
 }
