@@ -253,7 +253,7 @@ int sbamd_plan_backward(sbamd_plan* plan, const void* grad_out_device, const voi
 /* The same backward with no device allocation: no counterpart in the
  * reference; for callers that cannot allocate (code under hipGraph capture,
  * tight training or serving loops). Its working arrays live in one static
- * device array of about 50 MiB per GPU, held until the process ends, sized
+ * device array of about 67 MiB per GPU, held until the process ends, sized
  * for sets of at most 65,536 PCFs and 524,288 records (larger sets:
  * SBAMD_ERR_INVALID). The plan itself is made beforehand. Every call on a GPU
  * shares that GPU's array (each clears what it accumulates into): calls

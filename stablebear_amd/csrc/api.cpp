@@ -22,7 +22,7 @@
 #include <unordered_map>
 
 // Working memory of sbamd_plan_backward_noalloc (layout in NOALLOC_* below).
-static __device__ __align__(256) unsigned char scratch[(size_t(1) << 19) * 100 + (size_t(1) << 16) * 8];
+static __device__ __align__(256) unsigned char scratch[(size_t(1) << 19) * 132 + 8 * 32 + (size_t(1) << 16) * 8];
 
 namespace sbamd
 {
@@ -114,8 +114,9 @@ namespace sbamd
     // (every GPU the library runs on holds its own copy until the process
     // ends), laid out for a set of at most NOALLOC_MAX_PCFS PCFs and
     // NOALLOC_MAX_RECORDS records: per record start time, value, source
-    // point, two merge-sort keys, Q, A (8 bytes each), one 32-byte event,
-    // owner and two merge-sort ids (4 bytes each); per PCF its weight sum.
+    // point, two merge-sort keys, Q, A (8 bytes each), two 32-byte events,
+    // owner and two merge-sort ids (4 bytes each); per PCF its weight sum;
+    // BWD_EVENT_PAD events of padding after the second event array.
     constexpr int64_t NOALLOC_MAX_RECORDS = int64_t(1) << 19;
     constexpr int64_t NOALLOC_MAX_PCFS = int64_t(1) << 16;
     constexpr size_t NOALLOC_KEY0 = 0;
@@ -126,7 +127,8 @@ namespace sbamd
     constexpr size_t NOALLOC_Q = NOALLOC_KEYS1 + 8 * NOALLOC_MAX_RECORDS;
     constexpr size_t NOALLOC_A = NOALLOC_Q + 8 * NOALLOC_MAX_RECORDS;
     constexpr size_t NOALLOC_EVENTS = NOALLOC_A + 8 * NOALLOC_MAX_RECORDS;
-    constexpr size_t NOALLOC_COL0 = NOALLOC_EVENTS + sizeof(BwdEvent) * NOALLOC_MAX_RECORDS;
+    constexpr size_t NOALLOC_EVENTS2 = NOALLOC_EVENTS + sizeof(BwdEvent) * NOALLOC_MAX_RECORDS;
+    constexpr size_t NOALLOC_COL0 = NOALLOC_EVENTS2 + sizeof(BwdEvent) * (NOALLOC_MAX_RECORDS + BWD_EVENT_PAD);
     constexpr size_t NOALLOC_IDS0 = NOALLOC_COL0 + 4 * NOALLOC_MAX_RECORDS;
     constexpr size_t NOALLOC_IDS1 = NOALLOC_IDS0 + 4 * NOALLOC_MAX_RECORDS;
     constexpr size_t NOALLOC_W = NOALLOC_IDS1 + 4 * NOALLOC_MAX_RECORDS;
@@ -723,6 +725,7 @@ namespace sbamd
       double* keys[2];
       int32_t* ids[2];
       BwdEvent* events;
+      BwdEvent* events2;
       double* q;
       double* a;
       double* w;
@@ -780,9 +783,9 @@ namespace sbamd
       B.w_f64 = pl.tv_f64; // FP64 sets keep FP64 pair weights
       B.val_f64 = pl.tv_f64;
       B.grad_f64 = grad_dtype == SBAMD_F64 ? 1 : 0;
-      // chunks of 64 partners: 16 KB (FP32) or 32 KB (FP64) of weights in
-      // LDS, so that 16-wave workgroups still fill a CU
-      B.ch = BWD_GROUP;
+      // chunks of 64 partners with FP32 pair weights, 32 with FP64: a lane
+      // keeps its row's weights of the chunk in 64 vector registers
+      B.ch = pl.tv_f64 ? BWD_GROUP / 2 : BWD_GROUP;
       B.roff = s->roff.as<int64_t>();
       B.rec = s->rec.ptr;
       B.perm = s->perm.as<int64_t>();
@@ -798,7 +801,7 @@ namespace sbamd
         B.ids[b] = wk.ids[b];
       }
       B.events = wk.events;
-      B.events2 = B.events; // chunks are the same groups of 64
+      B.events2 = wk.events2;
       B.grad_out = grad_out_device;
       B.fwd_out = fwd_out_device;
       B.q = wk.q;
@@ -2179,6 +2182,7 @@ extern "C"
           plan->bwd_ids[b] = DeviceBuffer(M * 4);
         }
         plan->bwd_events = DeviceBuffer(M * sizeof(BwdEvent));
+        plan->bwd_events2 = DeviceBuffer((M + BWD_EVENT_PAD) * sizeof(BwdEvent));
         plan->bwd_q = DeviceBuffer(M * 8);
         plan->bwd_a = DeviceBuffer(M * 8);
         plan->bwd_w = DeviceBuffer(N * 8);
@@ -2194,6 +2198,7 @@ extern "C"
         wk.ids[b] = plan->bwd_ids[b].as<int32_t>();
       }
       wk.events = plan->bwd_events.as<BwdEvent>();
+      wk.events2 = plan->bwd_events2.as<BwdEvent>();
       wk.q = plan->bwd_q.as<double>();
       wk.a = plan->bwd_a.as<double>();
       wk.w = plan->bwd_w.as<double>();
@@ -2229,6 +2234,7 @@ extern "C"
       wk.q = reinterpret_cast<double*>(base + NOALLOC_Q);
       wk.a = reinterpret_cast<double*>(base + NOALLOC_A);
       wk.events = reinterpret_cast<BwdEvent*>(base + NOALLOC_EVENTS);
+      wk.events2 = reinterpret_cast<BwdEvent*>(base + NOALLOC_EVENTS2);
       wk.col0 = reinterpret_cast<int32_t*>(base + NOALLOC_COL0);
       wk.ids[0] = reinterpret_cast<int32_t*>(base + NOALLOC_IDS0);
       wk.ids[1] = reinterpret_cast<int32_t*>(base + NOALLOC_IDS1);

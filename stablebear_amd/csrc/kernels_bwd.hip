@@ -11,18 +11,38 @@
 // with W = sum_j w_ij, Q_r = integral of H_i over record r, A_r = H_i at
 // t_end(r), and H_i = sum_j w_ij g_j.
 //
-// H_i is piecewise constant and changes only where a partner has a
-// breakpoint, by w_ij times that partner's jump. PCFs are taken in groups of
-// 64 consecutive ones; each group's record starts, merged into one
-// time-ordered event list, serve twice. A wavefront handles a band of 64
-// rows (one lane per row) against a chunk of 64 partners: it walks the
-// chunk's list, where every event moves all lanes' H_i (w_ij from LDS), and
-// before each event every lane emits the Q and A of its own records that
-// end before that event's time, all lanes at once (the band's list, sorted
-// into per-lane queues in LDS, supplies each lane's breakpoints). The cost
-// is the chunk list's length plus one emission round per distinct gap
-// holding breakpoints; the chunks of a band add their Q and A with global
-// atomics, 64 records at a time.
+// With F_j(t) = integral of g_j over [0, t) and S_i(t) = sum_j w_ij F_j(t),
+// the integral of H_i from 0, Q_r = S_i(t_end(r)) - S_i(t_start(r)). So the
+// backward needs S_i and H_i at every breakpoint of every PCF, each a sum
+// over the partners. On a record of g_j, F_j(t) = P_j + f_j t (f_j its
+// value, P_j = F_j(t_start) - f_j t_start); a partner breakpoint at time t_e
+// that moves f_j by d moves P_j by -d t_e.
+//
+// PCFs are taken in groups of 64 ("bands") and of CH partners ("chunks": 64
+// with FP32 pair weights, 32 with FP64). A workgroup handles one band against
+// a group of two chunks. Its wavefronts split the band's breakpoints (one
+// time-ordered list) into slices and take them 64 at a time, one breakpoint
+// t_b (of row i_b) per lane. For a block whose first breakpoint is at T0, a
+// lane starts, chunk after chunk, from the partners' state at T0 (f_j and P_j
+// of each partner, in LDS):
+//   X = sum_j w_ij f_j(T0),  Y = -sum_j w_ij P_j(T0),
+// and walks the partner breakpoints from T0 to the block's last breakpoint:
+// each one with t_e < t_b adds w d to X and w d t_e to Y, so that at the end
+// X = H_i(t_b^-) and S_i(t_b) = t_b X - Y. The walk reads the events through
+// the scalar cache (they are the same for every lane), four at a time with
+// the next four in flight, and takes the lane's w_ij from a vector-register
+// array indexed by the event's partner; the widest array the compiler
+// indexes with a scalar is 32 registers, so a chunk's partners come as two
+// sublists (two event lists) of 32 (FP64 weights: 16). The same events then
+// move the partners' state on to the next block's T0: for each partner, the
+// last of its events before that time carries its value and P.
+//
+// Each breakpoint adds S_i(t_b) (summed over the group's partners) to q at
+// the record it starts and H_i(t_b) to a at the record it ends; the finalize
+// kernel takes Q_r = q[r + 1] - q[r], with S = 0 at the start of a first
+// record and, for a last record, S at the end of the domain (sum_j w_ij
+// F_j(DBL_MAX), kept in a[] at that record, where no breakpoint writes). The
+// chunk groups of a band add their parts with global atomics.
 //
 // All arrays here are in the original PCF order ("position" p of record k of
 // PCF i is ooff[i] + k); a layout kernel copies them out of the sorted
@@ -32,7 +52,7 @@
 // mean of H_i's left and right limits there (the mean of the partner's values
 // on both sides). For a self pair (kernel diagonal) that is the exact
 // derivative; for two PCFs jumping together the two one-sided results add up
-// to the derivative of moving the shared time.
+// to the derivative of moving the shared time. S is continuous there.
 #include "walk_common.hpp"
 
 namespace sbamd
@@ -85,64 +105,216 @@ namespace sbamd
       return ((const __attribute__((address_space(4))) T*)(p))[i];
     }
 
-    __device__ __forceinline__ BwdEvent uniform_event(const BwdEvent* p, int64_t i)
+    // Records at positions [a, b) of one PCF that start before t. Eight
+    // independent probes per round (the keys are sorted), so a search takes
+    // about log8 of the length in dependent loads.
+    __device__ __forceinline__ int64_t starts_below(const double* key0, int64_t a, int64_t b, double t)
+    {
+      int64_t lo = a, hi = b; // the count (relative to a at the end) is in [lo, hi]
+      while (hi - lo > 8)
+      {
+        const int64_t step = (hi - lo + 7) / 8;
+        int below = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+        {
+          const int64_t idx = lo + k * step;
+          below += idx < hi && key0[idx] < t;
+        }
+        // the probes are in key order: the first `below` of them lie below t
+        const int64_t nlo = below > 0 ? lo + (below - 1) * step + 1 : lo;
+        const int64_t nhi = below > 0 ? min(hi, lo + below * step) : lo;
+        lo = nlo;
+        hi = nhi;
+      }
+      int below = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        below += lo + k < hi && key0[lo + k] < t;
+      return lo + below - a;
+    }
+
+    // First index of a time-ordered event list of n events whose time is
+    // >= t, found by the whole wavefront (every lane active): 64 probes per
+    // round, so two or three dependent loads for a list of thousands.
+    __device__ __forceinline__ int32_t lower_bound_time_wave(const BwdEvent* p, int32_t n, double t)
+    {
+      const int lane = __lane_id();
+      int32_t lo = 0, hi = n; // the answer is in [lo, hi]
+      while (hi - lo > 64)
+      {
+        const int32_t step = (hi - lo + 63) / 64;
+        const int32_t idx = lo + lane * step;
+        const bool below = idx < hi && p[idx].t < t;
+        const int32_t cnt = static_cast<int32_t>(__popcll(__ballot(below)));
+        const int32_t nlo = cnt > 0 ? lo + (cnt - 1) * step + 1 : lo;
+        const int32_t nhi = cnt > 0 ? min(hi, lo + cnt * step) : lo;
+        lo = nlo;
+        hi = nhi;
+      }
+      const bool below = lo + lane < hi && p[lo + lane].t < t;
+      return __builtin_amdgcn_readfirstlane(lo + static_cast<int32_t>(__popcll(__ballot(below))));
+    }
+
+    // 1 if a < b, for times held in scalar registers as their bit patterns
+    // (non-negative doubles and +inf order like unsigned integers): the
+    // borrow of a 64-bit subtract, on the scalar unit.
+    __device__ __forceinline__ int32_t bits_before(uint64_t a, uint64_t b)
+    {
+      uint32_t tmp, r;
+      asm volatile("s_sub_u32 %0, %2, %4\n\t"
+                   "s_subb_u32 %0, %3, %5\n\t"
+                   "s_cselect_b32 %1, 1, 0"
+                   : "=&s"(tmp), "=s"(r)
+                   : "s"(static_cast<uint32_t>(a)), "s"(static_cast<uint32_t>(a >> 32)),
+                     "s"(static_cast<uint32_t>(b)), "s"(static_cast<uint32_t>(b >> 32))
+                   : "scc");
+      return static_cast<int32_t>(r);
+    }
+
+    // A partner event as the walk reads it: one 32-byte scalar load.
+    struct WalkEv
+    {
+      double t, d, dt;
+      int32_t col, pos;
+    };
+    static_assert(sizeof(WalkEv) == sizeof(BwdEvent), "the walk reads whole events");
+
+    __device__ __forceinline__ WalkEv walk_load(const BwdEvent* cl, int32_t i)
     {
       const __attribute__((address_space(4))) BwdEvent* c =
-          (const __attribute__((address_space(4))) BwdEvent*)(p);
-      BwdEvent e;
+          (const __attribute__((address_space(4))) BwdEvent*)(cl);
+      WalkEv e;
       e.t = c[i].t;
-      e.dt = c[i].dt;
       e.d = c[i].d;
+      e.dt = c[i].dt;
       e.col = c[i].col;
       e.pos = c[i].pos;
       return e;
     }
 
-    __device__ __forceinline__ double uniform_time(const BwdEvent* p, int32_t i)
+    // One partner event (time t, jump d, dt = d t, in scalar registers) for
+    // the lane's breakpoint tb and pair weight w: a lane with t < tb adds
+    // w d to X and w dt to Y; lanes with t == tb are set in `tie`. Written
+    // out because the compiler turns the masked update into selects of both
+    // FP64 results (four instructions more per event).
+    __device__ __forceinline__ void walk_step(double& X, double& Y, uint64_t& tie, double tb, float w,
+                                              double t, double d, double dt)
     {
-      return ((const __attribute__((address_space(4))) BwdEvent*)(p))[i].t;
+      float wm;
+      double wd;
+      asm volatile("v_cmp_lt_f64 vcc, %[t], %[tb]\n\t"
+                   "s_nop 1\n\t"
+                   "v_cndmask_b32 %[wm], 0, %[w], vcc\n\t"
+                   "v_cvt_f64_f32 %[wd], %[wm]\n\t"
+                   "v_cmp_eq_f64 vcc, %[t], %[tb]\n\t"
+                   "v_fmac_f64 %[X], %[d], %[wd]\n\t"
+                   "v_fmac_f64 %[Y], %[dt], %[wd]\n\t"
+                   "s_or_b64 %[tie], %[tie], vcc"
+                   : [X] "+v"(X), [Y] "+v"(Y), [tie] "+s"(tie), [wm] "=&v"(wm), [wd] "=&v"(wd)
+                   : [t] "s"(t), [tb] "v"(tb), [w] "v"(w), [d] "s"(d), [dt] "s"(dt)
+                   : "vcc", "scc");
     }
 
-    // First index of a time-ordered event list of n events whose time is >= t.
-    __device__ __forceinline__ int32_t lower_bound_time(const BwdEvent* p, int32_t n, double t)
+    __device__ __forceinline__ void walk_step(double& X, double& Y, uint64_t& tie, double tb, double w,
+                                              double t, double d, double dt)
     {
-      int32_t lo = 0, hi = n;
-      while (lo < hi)
-      {
-        const int32_t mid = (lo + hi) >> 1;
-        if (uniform_time(p, mid) < t)
-          lo = mid + 1;
+      const bool lt = t < tb;
+      const double wm = lt ? w : 0.0;
+      X = fma(wm, d, X);
+      Y = fma(wm, dt, Y);
+      tie |= __ballot(t == tb);
+    }
+
+    template <typename WT>
+    struct WalkVec;
+    template <>
+    struct WalkVec<float>
+    {
+      static constexpr int VW = 32;
+      typedef float type __attribute__((ext_vector_type(32)));
+    };
+    template <>
+    struct WalkVec<double>
+    {
+      static constexpr int VW = 16;
+      typedef double type __attribute__((ext_vector_type(16)));
+    };
+
+    // Walks the partner events of one sublist from index e while their time
+    // (bits) is below `lim` (bits), for the lane's breakpoint tb; returns the
+    // first index not taken. Events come four at a time through the scalar
+    // cache from one base address (the event arrays have four events of
+    // padding, so a group may run past the list); the next four are
+    // requested after the current ones have been waited for, so that they
+    // arrive while the current four are applied.
+    template <typename WT>
+    __device__ __forceinline__ int32_t walk_window(const BwdEvent* cl, int32_t n, int32_t e, uint64_t lim,
+                                                   const typename WalkVec<WT>::type& wv, double tb,
+                                                   double& X, double& Y, uint64_t& tie)
+    {
+      constexpr int VW = WalkVec<WT>::VW;
+      if (e >= n)
+        return e;
+      WalkEv A[4], Bq[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        A[u] = walk_load(cl + e, u);
+      auto half = [&](WalkEv(&cur)[4], WalkEv(&nxt)[4]) -> bool {
+        // events are in time order: all four are in the window when the
+        // fourth is (and the list has four more), else count them
+        int32_t k;
+        if (bits_before(static_cast<uint64_t>(__double_as_longlong(cur[3].t)), lim) && n - e >= 4)
+          k = 4;
         else
-          hi = mid;
+        {
+          // only events of this list count (those past its end belong to
+          // another list or are padding)
+          const int32_t nv = n - e;
+          k = 0;
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            k += u < nv ? bits_before(static_cast<uint64_t>(__double_as_longlong(cur[u].t)), lim) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          nxt[u] = walk_load(cl + e + 4, u);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (u < k)
+            walk_step(X, Y, tie, tb, wv[cur[u].col & (VW - 1)], cur[u].t, cur[u].d, cur[u].dt);
+        e += k;
+        return k == 4;
+      };
+      while (half(A, Bq) && half(Bq, A))
+      {
       }
-      return lo;
-    }
-
-    // Records at positions [a, b) of one PCF that start before t.
-    __device__ __forceinline__ int64_t starts_below(const double* key0, int64_t a, int64_t b, double t)
-    {
-      return count_before(key0, a, b, t, false) - a;
+      return e;
     }
   }
 
-  constexpr int BWD_WAVES = 16; // wavefronts per backward workgroup (time slices of one task)
+  // Wavefronts per backward workgroup (slices of one task's breakpoints):
+  // with 91 vector registers a lane, two 8-wave workgroups share a CU, so
+  // one's set-up (weights, searches) overlaps the other's walk.
+  constexpr int BWD_WAVES = 8;
 
-  // Per wave LDS: four arrays of 64 8-byte words (row queue times, pending
-  // Q, pending A, row masks) and three of 64 ints (row queue positions,
-  // pending records, queue offsets). The partners' start values (ch FP64,
-  // needed only before the main loop) share the first two 8-byte arrays.
-  constexpr size_t BWD_WAVE_LDS = 4 * 64 * sizeof(double) + 3 * 64 * sizeof(int32_t);
-  static_assert(BWD_WAVE_LDS >= 128 * sizeof(double), "start values of 128 partners must fit");
+  // Chunks of partners per workgroup: a breakpoint's two atomics are paid
+  // once per group of chunks (measured on 4096 PCFs: 1, 2 and 4 chunks take
+  // 1.83, 1.61 and 1.73 ms; four leave too few workgroups).
+  constexpr int BWD_CHUNKS = 2;
 
-  // LDS bytes of the pair weights (ch partners x 64 rows) and of the whole
-  // workgroup.
+  // LDS of the gather kernel, per chunk of the group: pair weights (ch
+  // partners x 64 rows), the partners' record end positions (ch ints), and
+  // per wave the partners' state (f, P) at the current block's start (ch x
+  // 16 bytes).
   __host__ __device__ constexpr size_t bwd_lds_weights(size_t wbytes, int ch)
   {
     return static_cast<size_t>(ch) * 64 * wbytes;
   }
   __host__ __device__ constexpr size_t bwd_lds_bytes(size_t wbytes, int ch)
   {
-    return bwd_lds_weights(wbytes, ch) + BWD_WAVES * BWD_WAVE_LDS;
+    return BWD_CHUNKS * (bwd_lds_weights(wbytes, ch) + static_cast<size_t>(ch) * 4 +
+                         BWD_WAVES * static_cast<size_t>(ch) * 16);
   }
 
   // Thread per sorted record: copies start time, value, owner and source
@@ -176,7 +348,7 @@ namespace sbamd
     B.ids[0][p] = static_cast<int32_t>(p);
   }
 
-  // One level of the per-chunk merge sort: runs of `width` consecutive PCFs
+  // One level of the per-group merge sort: runs of `width` consecutive PCFs
   // (each PCF's records are already in time order) merge pairwise; equal
   // times keep the left run first, so the order is (time, PCF, record).
   __global__ void bwd_merge_kernel(BwdParams B, int32_t width, int32_t from, int32_t group)
@@ -205,7 +377,8 @@ namespace sbamd
     B.ids[from ^ 1][pos] = iin[e];
   }
 
-  // Thread per position of the sorted chunk lists: the event record.
+  // Thread per position of the sorted group lists (groups of `group` PCFs):
+  // the event record (time, jump, jump x time, PCF within the group, record).
   __global__ void bwd_events_kernel(BwdParams B, int32_t from, int32_t group, BwdEvent* out)
   {
     const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -213,342 +386,323 @@ namespace sbamd
       return;
     const int64_t cb = (static_cast<int64_t>(B.col0[p]) / group) * group;
     const double t = B.keys[from][p];
-    const double tp = p > B.ooff[cb] ? B.keys[from][p - 1] : 0.0;
     const int64_t id = B.ids[from][p];
     const int64_t c = B.col0[id];
     const bool first = id == B.ooff[c];
     BwdEvent ev;
     ev.t = t;
-    ev.dt = t - tp;
     ev.d = B.val0[id] - (first ? 0.0 : B.val0[id - 1]);
+    ev.dt = ev.d * t;
     ev.col = static_cast<int32_t>(c - cb) | (first ? BWD_FIRST : 0);
     ev.pos = static_cast<int32_t>(id);
     out[p] = ev;
   }
 
-  // One workgroup per (band of 64 rows, chunk of CH partners), BWD_WAVES
-  // wavefronts splitting the task's time axis: wave k takes the events in
-  // [T_k, T_{k+1}), with T_k the time of the chunk's event at k/BWD_WAVES of
-  // its list. A wave starts from H_i just before T_k (from the partners'
-  // values there) and from the row record open there; the part of that
-  // record's integral before T_k was added by wave k - 1. Each wave holds
-  // blocks of 64 events of both lists in vector registers (lane l holds
-  // event l) and reads them with readlane. A breakpoint event costs one lane's work and a partner event
-  // all lanes', so wider chunks (CH = 128 with FP32 weights) spend a smaller
-  // share of the events on breakpoints. Workgroups are dealt to the 8 XCDs
-  // round-robin; all bands of a chunk run on one XCD so the chunk's list
-  // stays in that XCD's L2. Chunks go to XCDs in snake order (0..7, 7..0,
-  // ...) so that every XCD gets a like share of light and heavy chunks.
-  template <typename WT, int OP, int CH>
-  __global__ __launch_bounds__(64 * BWD_WAVES) void bwd_stream_kernel(BwdParams B)
+  // Wavefront per PCF: pst0[p] = F(t_start(p)) - v_p t_start(p), with F the
+  // PCF's integral from 0 (an inclusive scan of the record integrals over
+  // the lanes, 64 records at a time).
+  __global__ void bwd_prefix_kernel(BwdParams B)
   {
-    extern __shared__ __align__(16) unsigned char smem[];
-    WT* const s_w = reinterpret_cast<WT*>(smem); // s_w[jj * 64 + ii] = w(row ii, partner jj)
-    unsigned char* const s_wave = smem + bwd_lds_weights(sizeof(WT), CH);
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    // the wave index, as a wave-uniform value, so that everything derived
-    // from it (slice bounds, list indices) stays in scalar registers
-    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    double* const s_qt = reinterpret_cast<double*>(s_wave + wv * BWD_WAVE_LDS); // row queue: times
-    double* const s_eq = s_qt + 64;                                              // pending Q
-    double* const s_ea = s_eq + 64;                                              // pending A
-    unsigned long long* const s_m = reinterpret_cast<unsigned long long*>(s_ea + 64); // row masks
-    int32_t* const s_qp = reinterpret_cast<int32_t*>(s_m + 64);                  // row queue: positions
-    int32_t* const s_ep = s_qp + 64;                                             // pending records
-    int32_t* const s_off = s_ep + 64;                                            // queue offsets
-    double* const s_g = s_qt; // partners' values at tlo (before the main loop only)
-    const int64_t nbands = (B.n + BWD_GROUP - 1) / BWD_GROUP;
-    const int64_t nchunks = (B.n + CH - 1) / CH;
-    const int64_t x = blockIdx.x;
-    const int64_t slot = x >> 3;
-    const int64_t g = slot / nbands;
-    const int64_t cr = ((g & 1) ? 7 - (x & 7) : (x & 7)) + 8 * g;
-    if (cr >= nchunks)
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (i >= B.n)
       return;
-    // PCFs are sorted by record count, so the last chunks and bands are the
-    // heaviest tasks: they go first
-    const int64_t c = nchunks - 1 - cr;
-    const int64_t band = nbands - 1 - slot % nbands;
-    const int64_t i = band * BWD_GROUP + lane;
-    const bool act = i < B.n;
-    const int64_t j0 = c * CH;
-    const int ncol = static_cast<int>(min(static_cast<int64_t>(CH), B.n - j0));
-    const int nrow = static_cast<int>(min(static_cast<int64_t>(BWD_GROUP), B.n - band * BWD_GROUP));
-
-    // pair weights, partner-major: the lanes read one row of 64 consecutive
-    // words per event
-    for (int k = tid; k < CH * 64; k += 64 * BWD_WAVES)
+    const int lane = threadIdx.x & 63;
+    const int64_t a = B.ooff[i], b = B.ooff[i + 1];
+    double carry = 0.0;
+    for (int64_t base = a; base < b; base += 64)
     {
-      const int jj = k >> 6, ii = k & 63;
-      const double w = jj < ncol && ii < nrow ? pair_weight<OP>(B, band * BWD_GROUP + ii, j0 + jj) : 0.0;
-      s_w[k] = static_cast<WT>(w);
-    }
-    __syncthreads();
-    if (wv == 0 && act)
-    {
-      double ws = 0.0;
-      for (int jj = 0; jj < ncol; ++jj)
-        ws += static_cast<double>(s_w[jj * 64 + lane]);
-      if (ws != 0.0)
-        atomicAdd(&B.wsum[i], ws);
-    }
-
-    // this wave's slice of both lists
-    const int64_t r0 = uniform_load(B.ooff, band * BWD_GROUP);
-    const int32_t nr = static_cast<int32_t>(uniform_load(B.ooff, band * BWD_GROUP + nrow) - r0);
-    const int64_t c0 = uniform_load(B.ooff, j0);
-    const int32_t ncv = static_cast<int32_t>(uniform_load(B.ooff, j0 + ncol) - c0);
-    const BwdEvent* const rl = B.events + r0;
-    const BwdEvent* const cl = B.events2 + c0;
-    const bool last = wv == BWD_WAVES - 1;
-    double tlo = 0.0, thi = __builtin_inf();
-    int32_t cb = 0, ce = ncv, rb = 0, re = nr;
-    if (wv > 0)
-    {
-      tlo = uniform_time(cl, static_cast<int32_t>((static_cast<int64_t>(wv) * ncv) / BWD_WAVES));
-      cb = lower_bound_time(cl, ncv, tlo);
-      rb = lower_bound_time(rl, nr, tlo);
-    }
-    if (!last)
-    {
-      thi = uniform_time(cl, static_cast<int32_t>((static_cast<int64_t>(wv + 1) * ncv) / BWD_WAVES));
-      ce = lower_bound_time(cl, ncv, thi);
-      re = lower_bound_time(rl, nr, thi);
-    }
-
-    // H_i just before tlo, from the partners' values there (lane = partner)
-    double H = 0.0;
-    if (wv > 0)
-    {
-      for (int jb = 0; jb < CH; jb += 64)
-      {
-        const int jj = jb + lane;
-        double gv = 0.0;
-        if (jj < ncol)
-        {
-          const int64_t a = B.ooff[j0 + jj];
-          const int64_t k = starts_below(B.key0, a, B.ooff[j0 + jj + 1], tlo);
-          if (k > 0)
-            gv = B.val0[a + k - 1];
-        }
-        s_g[jj] = gv;
-      }
-      for (int jj = 0; jj < ncol; ++jj)
-        H = fma(static_cast<double>(s_w[jj * 64 + lane]), s_g[jj], H);
-    }
-    // the row record open just before tlo
-    int32_t open = 0; // positions are below 2^31 (checked when the plan is made)
-    if (act)
-    {
-      const int64_t a = B.ooff[i];
-      const int64_t k = wv > 0 ? starts_below(B.key0, a, B.ooff[i + 1], tlo) : 0;
-      open = static_cast<int32_t>(a + (k > 0 ? k - 1 : 0));
-    }
-    // past the chunk's last event H_i is sum_j w_ij (last value of g_j),
-    // taken exactly there rather than as the running sum of the jumps
-    double Hend = 0.0;
-    if (last)
-      for (int jj = 0; jj < ncol; ++jj)
-        Hend = fma(static_cast<double>(s_w[jj * 64 + lane]),
-                   B.val0[uniform_load(B.ooff, j0 + jj + 1) - 1], Hend);
-
-    // A block of 64 events of a list in vector registers: lane l holds
-    // event s + l (index clamped into the list, so that every register is
-    // defined). Only the fields used are loaded: a load whose destination is
-    // never read leaves a register that the compiler reuses, and that reuse
-    // waits for the load.
-    struct PartnerBlk
-    {
-      double t, d;
-      int32_t col;
-    };
-    auto load_partners = [&](int32_t s0) -> PartnerBlk {
-      const BwdEvent* const ev = cl + max(min(s0 + lane, ncv - 1), 0);
-      PartnerBlk b;
-      b.t = ev->t;
-      b.d = ev->d;
-      b.col = ev->col;
-      return b;
-    };
-    auto lane_d = [](double v, int l) -> double {
-      return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
-                              __builtin_amdgcn_readlane(__double2loint(v), l));
-    };
-
-    // Row breakpoints are taken by their own lanes, all lanes at once. The
-    // wave holds a block of up to 64 events of the band's list, sorted by
-    // lane into per-lane queues in LDS (s_qt / s_qp, lane L's events at
-    // [qptr, qend) in time order): a 64-bit mask per lane (bit e set when
-    // event e of the block ends one of its records, built with LDS atomic
-    // ORs) gives each event its rank in its lane's queue. nb is the lane's
-    // next breakpoint and npos the position of the record that starts
-    // there; a lane with no events left in the block has nb = the block's
-    // last time, so that a partner event past that time brings in the next
-    // block. The row block is loaded when it is needed, not ahead: a block
-    // held in registers across the partner loop costs register copies there
-    // that wait on every outstanding memory operation.
-    int32_t rs = rb;
-    int32_t qptr = 0, qend = 0;
-    double nb = 0.0, blast = 0.0;
-    int32_t npos = 0;
-    auto next_own = [&]() {
-      // qptr = 64 (a lane that had all 64 events) reads s_eq[0] / s_ep[0],
-      // still this wave's LDS, and the value is discarded
-      const double t = s_qt[qptr];
-      npos = s_qp[qptr];
-      nb = qptr < qend ? t : blast;
-    };
-    auto open_block = [&]() {
-      const BwdEvent* const ev = rl + max(min(rs + lane, nr - 1), 0);
-      const double t = ev->t;
-      const int32_t col = ev->col & 63, pos = ev->pos;
-      const bool valid = rs + lane < re && !(ev->col & BWD_FIRST);
-      s_m[lane] = 0ull;
-      __builtin_amdgcn_wave_barrier();
-      if (valid)
-        atomicOr(&s_m[col], 1ull << lane);
-      __builtin_amdgcn_wave_barrier();
-      // queue offsets: exclusive prefix sum of the lanes' event counts
-      const int32_t cnt = __popcll(s_m[lane]);
-      int32_t incl = cnt;
+      const int64_t p = base + lane;
+      const bool in = p < b;
+      const double k = in ? B.key0[p] : 0.0;
+      const double v = in ? B.val0[p] : 0.0;
+      const double kn = p + 1 < b ? B.key0[p + 1] : k;
+      const double c = v * (kn - k);
+      double incl = c;
       for (int o = 1; o < 64; o <<= 1)
       {
-        const int32_t v = __shfl_up(incl, o);
+        const double u = __shfl_up(incl, o);
         if (lane >= o)
-          incl += v;
+          incl += u;
       }
-      qptr = incl - cnt;
-      qend = incl;
-      s_off[lane] = qptr;
-      __builtin_amdgcn_wave_barrier();
-      if (valid)
-      {
-        const int32_t slot = s_off[col] + static_cast<int32_t>(__popcll(s_m[col] & ((1ull << lane) - 1ull)));
-        s_qt[slot] = t;
-        s_qp[slot] = pos;
-      }
-      __builtin_amdgcn_wave_barrier();
-      blast = rs + 64 < re ? lane_d(t, 63) : __builtin_inf();
-      next_own();
-    };
-    open_block();
-
-    // Pending emissions (record, Q, A), added to q and a 64 at a time.
-    int32_t nem = 0;
-    auto flush = [&]() {
-      __builtin_amdgcn_wave_barrier();
-      if (lane < nem)
-      {
-        atomicAdd(&B.q[s_ep[lane]], s_eq[lane]);
-        atomicAdd(&B.a[s_ep[lane]], s_ea[lane]);
-      }
-      nem = 0;
-    };
-
-    // next chunk event; blocks start at cb + 64 k, so event ic is lane
-    // (ic - cb) & 63 of its block
-    int32_t ic = cb;
-    PartnerBlk Cc = load_partners(cb);
-    __builtin_amdgcn_s_waitcnt(0x0F70); // vmcnt(0): see the block loads below
-    double acc = 0.0;   // integral of H_i from the open record's start (or tlo) to tprev
-    double tprev = tlo; // time of the chunk's last event taken
-
-    // Sum over the chunk's events at time tprev (all taken) of w_ij * jump:
-    // at a breakpoint there, A is the mean of H_i's two limits, H - J / 2.
-    auto jumps_at_tprev = [&](int32_t ic, double tprev) -> double {
-      double J = 0.0;
-      for (int32_t e = ic - 1; e >= cb; --e)
-      {
-        const BwdEvent T = uniform_event(cl, e);
-        if (__double_as_longlong(T.t) != __double_as_longlong(tprev))
-          break;
-        J = fma(static_cast<double>(s_w[(T.col & (CH - 1)) * 64 + lane]), T.d, J);
-      }
-      return J;
-    };
-    // Emission rounds (every lane with a breakpoint before T emits one
-    // record per round) and new row blocks, until no lane is below T.
-    auto drain = [&](double T) {
-      while (__ballot(nb < T))
-      {
-        const bool go = qptr < qend && nb < T;
-        const unsigned long long gom = __ballot(go);
-        if (gom)
-        {
-          const int32_t ngo = __popcll(gom);
-          if (nem + ngo > 64)
-            flush();
-          if (go)
-          {
-            // record npos - 1 ends at nb
-            double J = 0.0;
-            if (__double_as_longlong(nb) == __double_as_longlong(tprev))
-              J = jumps_at_tprev(ic, tprev);
-            const double d = nb - tprev;
-            const int32_t slot = nem + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(gom >> 32),
-                                                                 __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(gom), 0u));
-            s_eq[slot] = fma(H, d, acc);
-            s_ea[slot] = fma(-0.5, J, H);
-            s_ep[slot] = npos - 1;
-            acc = -H * d;
-            open = npos;
-            ++qptr;
-            next_own();
-          }
-          nem += ngo;
-        }
-        else
-        {
-          // the lanes below T are done with this block, which ends before T
-          rs += 64;
-          open_block();
-        }
-      }
-    };
-    double Ct = lane_d(Cc.t, 0);
-    while (ic < ce)
-    {
-      // breakpoints before the chunk's event ic see H before it (at equal
-      // times the chunk's events come first)
-      if (__ballot(nb < Ct))
-        drain(Ct);
-      const int l = (ic - cb) & 63;
-      const double Cd = lane_d(Cc.d, l);
-      const int32_t Ccol = __builtin_amdgcn_readlane(Cc.col, l) & (CH - 1);
-      const double w = static_cast<double>(s_w[Ccol * 64 + lane]);
-      acc = fma(H, Ct - tprev, acc);
-      H = fma(w, Cd, H);
-      tprev = Ct;
-      if (++ic < ce)
-      {
-        if (l == 63)
-        {
-          // The next block is loaded when needed and waited for here, once
-          // per block: a block still in flight where the paths of this loop
-          // meet would make the compiler wait on it (and on every atomic
-          // issued before it) at each event. 0x0F70 = vmcnt(0) alone.
-          Cc = load_partners(ic);
-          __builtin_amdgcn_s_waitcnt(0x0F70);
-        }
-        Ct = lane_d(Cc.t, (ic - cb) & 63);
-      }
+      const double excl = incl - c;
+      if (in)
+        B.pst0[p] = (carry + excl) - v * k;
+      carry += __shfl(incl, 63);
     }
-    // past the chunk's last event H_i is exact
-    if (last)
-      H = Hend;
-    drain(__builtin_inf());
-    flush();
-    if (act)
+  }
+
+  // One workgroup per (band of 64 rows, group of BWD_CHUNKS chunks of CH
+  // partners); the design is in the file comment. A lane's breakpoint
+  // collects X, Y and the tie jumps over all chunks of the group before its
+  // two atomics, so a breakpoint costs two atomics per group rather than per
+  // chunk. Workgroups are dealt to the 8 XCDs round-robin; all bands of a
+  // chunk group run on one XCD so the group's lists stay in that XCD's L2,
+  // and groups go to XCDs in snake order (0..7, 7..0, ...).
+  template <typename WT, int OP>
+  __global__ __launch_bounds__(64 * BWD_WAVES) void bwd_gather_kernel(BwdParams B)
+  {
+    constexpr int VW = WalkVec<WT>::VW;
+    constexpr int CH = 2 * VW;
+    constexpr int G = BWD_CHUNKS;
+    using WV = typename WalkVec<WT>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    // s_w[g][jj * 64 + ii] = w(row ii, partner jj of chunk g)
+    WT* const s_w = reinterpret_cast<WT*>(smem);
+    int32_t* const s_oend = reinterpret_cast<int32_t*>(smem + G * bwd_lds_weights(sizeof(WT), CH));
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // this wave's partner states (f, P), CH per chunk of the group
+    double2* const s_fp =
+        reinterpret_cast<double2*>(smem + G * bwd_lds_weights(sizeof(WT), CH) + G * CH * 4) + wv * G * CH;
+    const int64_t nbands = (B.n + BWD_GROUP - 1) / BWD_GROUP;
+    const int64_t nchunks = (B.n + CH - 1) / CH;
+    const int64_t ngroups = (nchunks + G - 1) / G;
+    const int64_t x = blockIdx.x;
+    const int64_t slot = x >> 3;
+    const int64_t gslot = slot / nbands;
+    const int64_t cr = ((gslot & 1) ? 7 - (x & 7) : (x & 7)) + 8 * gslot;
+    if (cr >= ngroups)
+      return;
+    const int64_t grp = ngroups - 1 - cr;
+    const int64_t band = nbands - 1 - slot % nbands;
+    const int nrow = static_cast<int>(min(static_cast<int64_t>(BWD_GROUP), B.n - band * BWD_GROUP));
+
+    // pair weights, partner-major, and the partners' record ends
+    for (int k = tid; k < G * CH * 64; k += 64 * BWD_WAVES)
     {
-      // the open record's integral up to thi; the last record runs to the
-      // end of the domain, DBL_MAX
-      double qv;
-      if (!last)
-        qv = fma(H, thi - tprev, acc);
+      const int g = k / (CH * 64);
+      const int jj = (k >> 6) % CH, ii = k & 63;
+      const int64_t jg = (grp * G + g) * CH + jj;
+      const double w = jg < B.n && ii < nrow ? pair_weight<OP>(B, band * BWD_GROUP + ii, jg) : 0.0;
+      s_w[k] = static_cast<WT>(w);
+    }
+    if (tid < G * CH)
+    {
+      const int64_t jg = grp * G * CH + tid;
+      s_oend[tid] = jg < B.n ? static_cast<int32_t>(B.ooff[jg + 1]) : 0;
+    }
+    __syncthreads();
+    if (wv == 0 && lane < nrow)
+    {
+      double ws = 0.0;
+      for (int k = 0; k < G * CH; ++k)
+        ws += static_cast<double>(s_w[k * 64 + lane]);
+      if (ws != 0.0)
+        atomicAdd(&B.wsum[band * BWD_GROUP + lane], ws);
+    }
+    if (wv == 1)
+    {
+      // each row's S at the end of the domain: sum_j w_ij (P_j + f_j DBL_MAX)
+      // over the partners' last records, kept at the row's last record
+      for (int k = lane; k < G * CH; k += 64)
+      {
+        double f = 0.0, P = 0.0;
+        if (grp * G * CH + k < B.n)
+        {
+          const int64_t lp = s_oend[k] - 1;
+          f = B.val0[lp];
+          P = B.pst0[lp];
+        }
+        s_fp[k] = make_double2(f, P);
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (lane < nrow)
+      {
+        double A = 0.0, Hend = 0.0;
+        for (int k = 0; k < G * CH; ++k)
+        {
+          const double w = static_cast<double>(s_w[k * 64 + lane]);
+          const double2 fp = s_fp[k];
+          A = fma(w, fp.y, A);
+          Hend = fma(w, fp.x, Hend);
+        }
+        const double Send = Hend != 0.0 ? fma(Hend, DBL_MAX, A) : A;
+        if (Send != 0.0)
+          atomicAdd(&B.a[B.ooff[band * BWD_GROUP + lane + 1] - 1], Send);
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+
+    // the band's breakpoints; the records starting at 0 (one per row, first
+    // in the list) are not breakpoints
+    const int64_t r0 = uniform_load(B.ooff, band * BWD_GROUP);
+    const int32_t nr = static_cast<int32_t>(uniform_load(B.ooff, band * BWD_GROUP + nrow) - r0);
+    const BwdEvent* const rl = B.events + r0;
+    int32_t skip;
+    {
+      const bool first = lane < nr && (rl[lane].col & BWD_FIRST) != 0;
+      const uint64_t m = ~__ballot(first);
+      skip = m ? static_cast<int32_t>(__builtin_ctzll(m)) : 64;
+      skip = __builtin_amdgcn_readfirstlane(min(skip, nr));
+    }
+    const int32_t span = nr - skip;
+    const int32_t b_lo = skip + static_cast<int32_t>((static_cast<int64_t>(span) * wv) / BWD_WAVES);
+    const int32_t b_hi = skip + static_cast<int32_t>((static_cast<int64_t>(span) * (wv + 1)) / BWD_WAVES);
+    if (b_lo >= b_hi)
+      return;
+
+    // chunk g's sublist s of partner events: cl(g, s)[0, cn(g, s))
+    auto list_of = [&](int g, int s, const BwdEvent*& lst, int32_t& cnt) {
+      const int64_t p0 = (grp * G + g) * CH + s * VW;
+      if (p0 < B.n)
+      {
+        const int64_t base = uniform_load(B.ooff, p0);
+        lst = B.events2 + base;
+        cnt = static_cast<int32_t>(uniform_load(B.ooff, min(p0 + VW, B.n)) - base);
+      }
       else
-        qv = H != 0.0 ? fma(H, DBL_MAX - tprev, acc) : acc;
-      atomicAdd(&B.q[open], qv);
+      {
+        lst = B.events2;
+        cnt = 0;
+      }
+    };
+
+    // partners' state just before the slice's first breakpoint
+    const double T0 = uniform_load(&rl[b_lo].t, 0);
+    for (int k = lane; k < G * CH; k += 64)
+    {
+      double f = 0.0, P = 0.0;
+      const int64_t jg = grp * G * CH + k;
+      if (jg < B.n)
+      {
+        const int64_t a = B.ooff[jg];
+        const int64_t c = starts_below(B.key0, a, s_oend[k], T0);
+        if (c > 0)
+        {
+          f = B.val0[a + c - 1];
+          P = B.pst0[a + c - 1];
+        }
+      }
+      s_fp[k] = make_double2(f, P);
+    }
+    // the walk's position in each list: lane 2 g + s of one vector register
+    int32_t ecv = 0;
+    for (int k = 0; k < 2 * G; ++k)
+    {
+      const BwdEvent* lst;
+      int32_t cnt;
+      list_of(k >> 1, k & 1, lst, cnt);
+      const int32_t e = cnt > 0 ? lower_bound_time_wave(lst, cnt, T0) : 0;
+      ecv = lane == k ? e : ecv;
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    for (int32_t blk = b_lo; blk < b_hi; blk += 64)
+    {
+      const int32_t bb = blk + lane;
+      const bool valid = bb < b_hi;
+      const BwdEvent& E = rl[min(bb, b_hi - 1)];
+      const double tb = valid ? E.t : -1.0; // an invalid lane takes no event
+      const int32_t ecol = E.col;
+      const int32_t pos = E.pos;
+      const int ib = ecol & 63;
+      const bool emit = valid && !(ecol & BWD_FIRST);
+      const bool lastblk = blk + 64 >= b_hi;
+      const double Tlast = uniform_load(&rl[min(blk + 63, b_hi - 1)].t, 0);
+      const double Tend = lastblk ? Tlast : uniform_load(&rl[blk + 64].t, 0);
+      // the walk takes events with t < Tend, or t <= Tlast in a slice's last block
+      uint64_t lim = static_cast<uint64_t>(__double_as_longlong(Tend)) + (lastblk ? 1u : 0u);
+      // (uniform by construction; stated for the compiler, whose analysis
+      // loses it through the loop)
+      lim = (static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(lim >> 32))) << 32) |
+            static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(lim)));
+      const bool edge = !lastblk && __ballot(valid && tb == Tend) != 0;
+
+      double X = 0.0, Y = 0.0, J = 0.0; // over the group's partners
+#pragma unroll 1
+      for (int g = 0; g < G; ++g)
+      {
+        if ((grp * G + g) * CH >= B.n)
+          break;
+        const WT* const w_g = s_w + g * CH * 64;
+        double2* const fp_g = s_fp + g * CH;
+        const int32_t* const oend_g = s_oend + g * CH;
+        const BwdEvent *cl0, *cl1;
+        int32_t cn0, cn1;
+        list_of(g, 0, cl0, cn0);
+        list_of(g, 1, cl1, cn1);
+        // partners at the block start, and the lane's row weights (two
+        // named vectors: an array of them would not stay in registers)
+        WV wv0, wv1;
+#pragma unroll
+        for (int v = 0; v < VW; ++v)
+        {
+          const WT w = w_g[v * 64 + ib];
+          wv0[v] = w;
+          const double2 fp = fp_g[v];
+          X = fma(static_cast<double>(w), fp.x, X);
+          Y = fma(-static_cast<double>(w), fp.y, Y);
+        }
+#pragma unroll
+        for (int v = 0; v < VW; ++v)
+        {
+          const WT w = w_g[(VW + v) * 64 + ib];
+          wv1[v] = w;
+          const double2 fp = fp_g[VW + v];
+          X = fma(static_cast<double>(w), fp.x, X);
+          Y = fma(-static_cast<double>(w), fp.y, Y);
+        }
+
+        uint64_t tie = 0;
+        const int32_t ws0 = __builtin_amdgcn_readlane(ecv, 2 * g);
+        const int32_t ws1 = __builtin_amdgcn_readlane(ecv, 2 * g + 1);
+        const int32_t we0 = walk_window<WT>(cl0, cn0, ws0, lim, wv0, tb, X, Y, tie);
+        const int32_t we1 = walk_window<WT>(cl1, cn1, ws1, lim, wv1, tb, X, Y, tie);
+        ecv = lane == 2 * g ? we0 : (lane == 2 * g + 1 ? we1 : ecv);
+
+        if (tie != 0 || edge)
+        {
+          // events inside the window at tb, and (edge) those at Tend just past it
+          for (int32_t e = ws0; e < cn0; ++e)
+          {
+            const WalkEv ev = walk_load(cl0, e);
+            if (e >= we0 && !(edge && ev.t == Tend))
+              break;
+            if (ev.t == tb)
+              J = fma(static_cast<double>(wv0[ev.col & (VW - 1)]), ev.d, J);
+          }
+          for (int32_t e = ws1; e < cn1; ++e)
+          {
+            const WalkEv ev = walk_load(cl1, e);
+            if (e >= we1 && !(edge && ev.t == Tend))
+              break;
+            if (ev.t == tb)
+              J = fma(static_cast<double>(wv1[ev.col & (VW - 1)]), ev.d, J);
+          }
+        }
+
+        if (!lastblk)
+        {
+          // move the partners' state to Tend: per partner, the last of its
+          // events in the window (the record that reaches Tend) sets it
+#pragma unroll
+          for (int s = 0; s < 2; ++s)
+          {
+            const BwdEvent* const lst = s ? cl1 : cl0;
+            const int32_t e1 = s ? we1 : we0;
+            for (int32_t e0 = s ? ws1 : ws0; e0 < e1; e0 += 64)
+            {
+              const int32_t e = e0 + lane;
+              if (e < e1)
+              {
+                const BwdEvent& ev = lst[e];
+                const int jj = s * VW + (ev.col & (VW - 1));
+                const int32_t p = ev.pos;
+                const double tn = p + 1 < oend_g[jj] ? B.key0[p + 1] : __builtin_inf();
+                if (tn >= Tend)
+                  fp_g[jj] = make_double2(B.val0[p], B.pst0[p]);
+              }
+            }
+          }
+        }
+      }
+      if (emit)
+      {
+        atomicAdd(&B.q[pos], fma(tb, X, -Y));
+        atomicAdd(&B.a[pos - 1], fma(0.5, J, X));
+      }
+      __builtin_amdgcn_wave_barrier();
     }
   }
 
@@ -565,10 +719,12 @@ namespace sbamd
     const int64_t first = B.ooff[i];
     const bool last = p + 1 == B.ooff[i + 1];
     const double v = B.val0[p];
+    // integral of H_i over the record: S at its end minus S at its start
+    const double Q = (last ? B.a[p] : B.q[p + 1]) - (p > first ? B.q[p] : 0.0);
     double gv, gt = 0.0;
     if (B.op == OPK_DOT)
     {
-      gv = B.q[p];
+      gv = Q;
       if (p > first)
         gt = (B.val0[p - 1] - v) * B.a[p - 1];
     }
@@ -577,7 +733,7 @@ namespace sbamd
       const double W = B.wsum[i];
       // the last record ends where the domain does, at DBL_MAX
       const double len = (last ? DBL_MAX : B.key0[p + 1]) - B.key0[p];
-      gv = 2.0 * ((v != 0.0 ? v * len * W : 0.0) - B.q[p]);
+      gv = 2.0 * ((v != 0.0 ? v * len * W : 0.0) - Q);
       if (p > first)
       {
         const double vp = B.val0[p - 1];
@@ -600,64 +756,62 @@ namespace sbamd
   {
     if (prm.records <= 0 || prm.n <= 0)
       return;
-    if (prm.ch != BWD_GROUP && !(prm.ch == 2 * BWD_GROUP && !prm.w_f64))
-      fail(SBAMD_ERR_RUNTIME, "backward: chunks of 64 partners, or 128 with FP32 weights");
+    const int32_t vw = prm.w_f64 ? WalkVec<double>::VW : WalkVec<float>::VW;
+    if (prm.ch != 2 * vw)
+      fail(SBAMD_ERR_RUNTIME, "backward: chunks of 64 partners (FP32 weights) or 32 (FP64)");
     const unsigned rgrid = static_cast<unsigned>((prm.records + 255) / 256);
     hipLaunchKernelGGL(bwd_layout_kernel, dim3(rgrid), dim3(256), 0, stream, prm);
     SBAMD_HIP(hipGetLastError());
-    // merge levels up to groups of 64 (the bands' lists), then one more
-    // level for chunks of 128
+    // merge levels up to groups of 64 (the bands' lists); the sublists of
+    // vw partners are read off at their level
     int from = 0;
-    for (int32_t width = 1; width < prm.ch; width *= 2)
+    for (int32_t width = 1; width < BWD_GROUP; width *= 2)
     {
-      const int32_t group = width < BWD_GROUP ? BWD_GROUP : prm.ch;
-      if (width == BWD_GROUP)
+      if (width == vw)
       {
-        hipLaunchKernelGGL(bwd_events_kernel, dim3(rgrid), dim3(256), 0, stream, prm, from, BWD_GROUP,
-                           prm.events);
+        hipLaunchKernelGGL(bwd_events_kernel, dim3(rgrid), dim3(256), 0, stream, prm, from, vw, prm.events2);
         SBAMD_HIP(hipGetLastError());
       }
-      hipLaunchKernelGGL(bwd_merge_kernel, dim3(rgrid), dim3(256), 0, stream, prm, width, from, group);
+      hipLaunchKernelGGL(bwd_merge_kernel, dim3(rgrid), dim3(256), 0, stream, prm, width, from, BWD_GROUP);
       SBAMD_HIP(hipGetLastError());
       from ^= 1;
     }
-    hipLaunchKernelGGL(bwd_events_kernel, dim3(rgrid), dim3(256), 0, stream, prm, from, prm.ch,
-                       prm.events2);
+    hipLaunchKernelGGL(bwd_events_kernel, dim3(rgrid), dim3(256), 0, stream, prm, from, BWD_GROUP, prm.events);
+    SBAMD_HIP(hipGetLastError());
+    // the sort keys are free now: pst0 takes the other buffer
+    BwdParams B = prm;
+    B.pst0 = prm.keys[from ^ 1];
+    const unsigned pgrid = static_cast<unsigned>((prm.n + 3) / 4);
+    hipLaunchKernelGGL(bwd_prefix_kernel, dim3(pgrid), dim3(256), 0, stream, B);
     SBAMD_HIP(hipGetLastError());
 
-    // (band, chunk) workgroups: every band of 64 rows against every chunk of
-    // ch partners, chunk-major per XCD
+    // (band, chunk group) workgroups: every band of 64 rows against every
+    // group of BWD_CHUNKS chunks of ch partners, group-major per XCD
     const int64_t nbands = (prm.n + BWD_GROUP - 1) / BWD_GROUP;
     const int64_t nchunks = (prm.n + prm.ch - 1) / prm.ch;
-    const int64_t grid = 8 * nbands * ((nchunks + 7) / 8);
+    const int64_t ngroups = (nchunks + BWD_CHUNKS - 1) / BWD_CHUNKS;
+    const int64_t grid = 8 * nbands * ((ngroups + 7) / 8);
     if (grid > 0x7fffffff)
       fail(SBAMD_ERR_INVALID, "backward: too many PCFs for one launch");
     const size_t lds = bwd_lds_bytes(prm.w_f64 ? 8 : 4, prm.ch);
-    const dim3 g(static_cast<unsigned>(grid));
+    const dim3 gr(static_cast<unsigned>(grid));
     const dim3 blk(64 * BWD_WAVES);
     if (prm.w_f64)
     {
       if (prm.op == OPK_DOT)
-        hipLaunchKernelGGL((bwd_stream_kernel<double, OPK_DOT, BWD_GROUP>), g, blk, lds, stream, prm);
+        hipLaunchKernelGGL((bwd_gather_kernel<double, OPK_DOT>), gr, blk, lds, stream, B);
       else
-        hipLaunchKernelGGL((bwd_stream_kernel<double, OPK_L2, BWD_GROUP>), g, blk, lds, stream, prm);
-    }
-    else if (prm.ch == 2 * BWD_GROUP)
-    {
-      if (prm.op == OPK_DOT)
-        hipLaunchKernelGGL((bwd_stream_kernel<float, OPK_DOT, 2 * BWD_GROUP>), g, blk, lds, stream, prm);
-      else
-        hipLaunchKernelGGL((bwd_stream_kernel<float, OPK_L2, 2 * BWD_GROUP>), g, blk, lds, stream, prm);
+        hipLaunchKernelGGL((bwd_gather_kernel<double, OPK_L2>), gr, blk, lds, stream, B);
     }
     else
     {
       if (prm.op == OPK_DOT)
-        hipLaunchKernelGGL((bwd_stream_kernel<float, OPK_DOT, BWD_GROUP>), g, blk, lds, stream, prm);
+        hipLaunchKernelGGL((bwd_gather_kernel<float, OPK_DOT>), gr, blk, lds, stream, B);
       else
-        hipLaunchKernelGGL((bwd_stream_kernel<float, OPK_L2, BWD_GROUP>), g, blk, lds, stream, prm);
+        hipLaunchKernelGGL((bwd_gather_kernel<float, OPK_L2>), gr, blk, lds, stream, B);
     }
     SBAMD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(finalize_grad_kernel, dim3(rgrid), dim3(256), 0, stream, prm);
+    hipLaunchKernelGGL(finalize_grad_kernel, dim3(rgrid), dim3(256), 0, stream, B);
     SBAMD_HIP(hipGetLastError());
   }
 }

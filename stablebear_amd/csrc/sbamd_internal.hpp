@@ -140,14 +140,15 @@ namespace sbamd
   // index within the group, with BWD_FIRST set on a PCF's first record.
   struct BwdEvent
   {
-    double t;
-    double dt;
-    double d;
-    int32_t col;
-    int32_t pos;
+    double t;   // record start time
+    double d;   // value jump at t (the first record: its value)
+    double dt;  // d * t
+    int32_t col; // PCF within its group (| BWD_FIRST for a first record)
+    int32_t pos; // record position
   };
   constexpr int32_t BWD_GROUP = 64;
   constexpr int32_t BWD_FIRST = 0x40000000;
+  constexpr int64_t BWD_EVENT_PAD = 8; // events readable past the end of the sublist array
   static_assert(sizeof(BwdEvent) == 32, "BwdEvent must stay 32 bytes");
 
   // Parameters of the backward kernels (kernels_bwd.hip). Arrays marked
@@ -156,7 +157,7 @@ namespace sbamd
   {
     int64_t n;       // PCFs
     int64_t records; // records of the set (= positions)
-    int32_t ch;      // partners per chunk (BWD_GROUP or 2 * BWD_GROUP)
+    int32_t ch;      // partners per chunk: 64 (FP32 pair weights) or 32 (FP64)
     int32_t op;      // OPK_L2 (pdist) or OPK_DOT (kernel with diagonal)
     int32_t in_f64;  // grad_out / fwd_out are FP64
     int32_t w_f64;   // pair weights kept in FP64 (else FP32) in LDS
@@ -176,12 +177,14 @@ namespace sbamd
     double* keys[2];     // merge sort ping-pong
     int32_t* ids[2];
     BwdEvent* events;  // event lists of the groups of BWD_GROUP PCFs (the bands)
-    BwdEvent* events2; // event lists of the chunks of ch PCFs (== events when ch == BWD_GROUP)
+    BwdEvent* events2; // event lists of the groups of ch / 2 PCFs (the chunks' sublists), with
+                       // BWD_EVENT_PAD events of padding after the last
+    double* pst0;      // per position: F(t_start) - v t_start (set inside launch_backward)
     // loss and accumulators
     const void* grad_out;
     const void* fwd_out;
-    double* q;    // per position: integral of H over the record
-    double* a;    // per position: H where the record ends
+    double* q;    // per position: S = integral of H from 0 to the record's start
+    double* a;    // per position: H where the record ends (last record: S at DBL_MAX)
     double* wsum; // per PCF: sum of its pair weights
     void* grad_v;
     void* grad_t;
@@ -307,7 +310,7 @@ namespace sbamd
     DeviceBuffer counter;
     // backward working arrays (kernels_bwd.hip), allocated on first use
     DeviceBuffer bwd_ooff, bwd_key0, bwd_val0, bwd_col0, bwd_src0, bwd_keys[2], bwd_ids[2],
-        bwd_events, bwd_q, bwd_a, bwd_w;
+        bwd_events, bwd_events2, bwd_q, bwd_a, bwd_w;
     // long-pair kernel: mode 0 off, 1 on, 2 auto (few long pairs, FP32C);
     // use_long is the decision for this plan
     int32_t long_mode = 2;
