@@ -1,4 +1,5 @@
-// Forward pairwise integration: one workgroup per pair tile, two pairs per lane.
+// Forward pairwise integration: one workgroup per pair tile, one or two pairs
+// per lane (the NCH template parameter).
 //
 // Replaces the reference's one-thread-per-pair kernel over global memory
 // (include/sbear/cuda/cuda_pcf_kernel.cuh:36-136). Here a workgroup owns an
@@ -12,10 +13,11 @@
 // The walk step is branch-free: one compare picks the side whose record ends
 // first, one LDS load fetches that side's next record, and selects update the
 // state, so a wavefront issues the same instruction stream for every lane.
-// Each step waits on its own LDS load, so a lane walks two pairs, (r, c) and
-// (r, c + ceil(C/2)), and issues both pairs' loads before it consumes either: two
-// independent chains per lane keep the SIMDs issuing while loads are in
-// flight.
+// Each step waits on its own LDS load; with NCH = 2 a lane walks two pairs,
+// (r, c) and (r, c + ceil(C/2)), and issues both pairs' loads before it
+// consumes either, so two independent chains per lane keep the SIMDs issuing
+// while loads are in flight (the 32 x 32 shapes the tuner picks for long
+// PCFs). With NCH = 1, more workgroups per CU hide the same latency.
 #include "walk_common.hpp"
 
 namespace sbamd
