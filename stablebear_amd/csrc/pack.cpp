@@ -10,6 +10,7 @@
 #include <limits>
 #include <numeric>
 #include <sstream>
+#include <memory>
 #include <unordered_map>
 
 namespace sbamd
@@ -182,17 +183,21 @@ namespace sbamd
       fail(SBAMD_ERR_INVALID, "symmetric plans need square tiles");
     const int64_t nbr = (nr + R - 1) / R;
     const int64_t nbc = (nc + C - 1) / C;
-    auto block_max = [](const std::vector<int64_t>& m, int64_t b0, int64_t cnt) {
-      int64_t mx = 0;
-      for (int64_t k = b0; k < b0 + cnt; ++k)
-        mx = std::max(mx, m[k]);
+    // longest PCF of every row and column block, once
+    auto block_maxima = [](const std::vector<int64_t>& m, int64_t n, int B, int64_t nb) {
+      std::vector<int64_t> mx(static_cast<size_t>(nb), 0);
+      for (int64_t k = 0; k < n; ++k)
+        mx[static_cast<size_t>(k / B)] = std::max(mx[static_cast<size_t>(k / B)], m[static_cast<size_t>(k)]);
       return mx;
     };
+    const std::vector<int64_t> rmx = block_maxima(m_rows, nr, R, nbr);
+    const std::vector<int64_t> cmx = sym ? rmx : block_maxima(mc, nc, C, nbc);
+    tiles.reserve(tiles.size() + static_cast<size_t>(sym ? nbr * (nbr + 1) / 2 : nbr * nbc));
     for (int64_t bi = nbr - 1; bi >= 0; --bi)
     {
       const int64_t r0 = bi * R;
       const int64_t rn = std::min<int64_t>(R, nr - r0);
-      const int64_t rmax = block_max(m_rows, r0, rn);
+      const int64_t rmax = rmx[static_cast<size_t>(bi)];
       const int64_t bj_end = sym ? bi : nbc - 1;
       for (int64_t bj = bj_end; bj >= 0; --bj)
       {
@@ -208,7 +213,7 @@ namespace sbamd
         t.nc = static_cast<int16_t>(cn);
         t.prob = prob;
         t.flags = diag ? TILE_DIAG : 0;
-        if (rmax + block_max(mc, c0, cn) - 1 >= fold_threshold)
+        if (rmax + cmx[static_cast<size_t>(bj)] - 1 >= fold_threshold)
           t.flags |= TILE_FOLD;
         tiles.push_back(t);
       }
@@ -279,39 +284,94 @@ namespace sbamd
   {
     if (nxcd <= 1 || b1 <= b0 + 1)
       return 0;
+    const size_t n = b1 - b0;
     auto key = [&](const Tile& t, int G) {
-      return (static_cast<uint64_t>(t.prob) << 48) | (static_cast<uint64_t>(t.r0 / (G * R)) << 24) |
-             static_cast<uint64_t>(t.c0 / (G * C));
+      const int sr = G * R, sc = G * C; // powers of two for the tile sides in use: shifts
+      const uint64_t r = (sr & (sr - 1)) == 0 ? static_cast<uint64_t>(t.r0) >> __builtin_ctz(sr)
+                                              : static_cast<uint64_t>(t.r0 / sr);
+      const uint64_t c = (sc & (sc - 1)) == 0 ? static_cast<uint64_t>(t.c0) >> __builtin_ctz(sc)
+                                              : static_cast<uint64_t>(t.c0 / sc);
+      return (static_cast<uint64_t>(t.prob) << 48) | (r << 24) | c;
     };
+    std::vector<uint32_t> gid(n);
     for (int G : {16, 8, 4})
     {
-      std::unordered_map<uint64_t, size_t> id;
-      std::vector<std::vector<Tile>> groups;
-      for (size_t k = b0; k < b1; ++k)
+      // supertile of every tile, numbered by first appearance (consecutive
+      // tiles mostly share one, so the map is consulted once per run)
+      std::unordered_map<uint64_t, uint32_t> id;
+      uint64_t last = ~0ull;
+      uint32_t last_id = 0;
+      for (size_t k = 0; k < n; ++k)
       {
-        auto it = id.find(key(tiles[k], G));
-        if (it == id.end())
+        const uint64_t kk = key(tiles[b0 + k], G);
+        if (kk != last)
         {
-          it = id.emplace(key(tiles[k], G), groups.size()).first;
-          groups.emplace_back();
+          auto it = id.emplace(kk, static_cast<uint32_t>(id.size())).first;
+          last = kk;
+          last_id = it->second;
         }
-        groups[it->second].push_back(tiles[k]);
+        gid[k] = last_id;
       }
-      if (groups.size() < 4 * static_cast<size_t>(nxcd))
+      const size_t ng = id.size();
+      if (ng < 4 * static_cast<size_t>(nxcd))
         continue;
-      std::vector<std::vector<Tile>> seq(static_cast<size_t>(nxcd));
-      for (size_t s = 0; s < groups.size(); ++s)
+      // supertiles in first-appearance order, tiles in list order within
+      std::vector<size_t> start(ng + 1, 0);
+      for (size_t k = 0; k < n; ++k)
+        ++start[gid[k] + 1];
+      for (size_t g = 0; g < ng; ++g)
+        start[g + 1] += start[g];
+      // XCD x takes supertiles x, x + nxcd, ...: the rank of a tile in its
+      // XCD's sequence is the tiles of that XCD's earlier supertiles plus its
+      // rank in its own supertile
+      const size_t nx = static_cast<size_t>(nxcd);
+      std::vector<size_t> seq_before(ng), seq_len(nx, 0);
+      for (size_t g = 0; g < ng; ++g)
       {
-        auto& q = seq[s % static_cast<size_t>(nxcd)];
-        q.insert(q.end(), groups[s].begin(), groups[s].end());
+        seq_before[g] = seq_len[g % nx];
+        seq_len[g % nx] += start[g + 1] - start[g];
       }
-      size_t pos = b0;
-      for (size_t k = 0; pos < b1; ++k)
-        for (auto& q : seq)
-          if (k < q.size())
-            tiles[pos++] = q[k];
+      // the launch order interleaves the XCDs' sequences: rank k of XCD x
+      // goes after every XCD's first k tiles (as far as they are that long)
+      // and the tiles of rank k of XCDs below x
+      size_t maxlen = 0;
+      for (size_t x = 0; x < nx; ++x)
+        maxlen = std::max(maxlen, seq_len[x]);
+      std::vector<size_t> slot_base(maxlen + 1, 0); // output position of rank k's first tile
+      {
+        std::vector<size_t> lens(seq_len);
+        std::sort(lens.begin(), lens.end());
+        size_t alive = nx, li = 0;
+        for (size_t k = 0; k < maxlen; ++k)
+        {
+          while (li < nx && lens[li] <= k)
+          {
+            ++li;
+            --alive;
+          }
+          slot_base[k + 1] = slot_base[k] + alive;
+        }
+      }
+      const size_t minlen = *std::min_element(seq_len.begin(), seq_len.end());
+      std::unique_ptr<Tile[]> out(new Tile[n]);
+      std::vector<size_t> fill(start.begin(), start.end() - 1);
+      for (size_t k = 0; k < n; ++k)
+      {
+        const size_t g = gid[k], x = g % nx;
+        const size_t r = seq_before[g] + (fill[g]++ - start[g]);
+        size_t below = x; // XCDs below x with a tile of rank r: all of them
+        if (r >= minlen)  // unless some sequences have ended
+        {
+          below = 0;
+          for (size_t y = 0; y < x; ++y)
+            below += seq_len[y] > r;
+        }
+        out[slot_base[r] + below] = tiles[b0 + k];
+      }
+      std::copy(out.get(), out.get() + n, tiles.begin() + static_cast<std::ptrdiff_t>(b0));
       return G;
     }
     return 0;
   }
+
 }
