@@ -292,6 +292,7 @@ namespace sbamd
   {
     unsigned char* smem;
     LdsLayout lay;
+    uint32_t base; // LDS address of smem
     unsigned long long* src;
     TT* tstart;
     int32_t* rem;
@@ -300,6 +301,11 @@ namespace sbamd
 
     __device__ TileLds(unsigned char* s, const LdsLayout& L) : smem(s), lay(L)
     {
+#if __HIP_DEVICE_COMPILE__
+      base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) unsigned char*)s));
+#else
+      base = 0;
+#endif
       src = reinterpret_cast<unsigned long long*>(s + L.src_off);
       tstart = reinterpret_cast<TT*>(s + L.tstart_off);
       rem = reinterpret_cast<int32_t*>(s + L.rem_off);
@@ -307,30 +313,37 @@ namespace sbamd
       tb = reinterpret_cast<typename Lim<TT>::Bits*>(s + L.tb_off);
     }
 
-    // LDS byte address of record 0 of slot s's window
+    // LDS address of record 0 of slot s's window. Walk pointers are LDS
+    // addresses (not offsets from smem), so a record read takes the pointer
+    // as its address with no add of the dynamic-LDS base in the loop.
     __device__ __forceinline__ uint32_t win(int s) const
     {
-      return static_cast<uint32_t>(lay.win_off) +
+      return base + static_cast<uint32_t>(lay.win_off) +
              static_cast<uint32_t>(s) * static_cast<uint32_t>(lay.sr * lay.rb);
     }
 
     __device__ __forceinline__ Rec<TT, TV>& rec(uint32_t addr) const
     {
-      return *reinterpret_cast<Rec<TT, TV>*>(smem + addr);
+#if __HIP_DEVICE_COMPILE__
+      using P3 = __attribute__((address_space(3))) unsigned char*;
+      return *reinterpret_cast<Rec<TT, TV>*>((unsigned char*)(P3)(uintptr_t)addr);
+#else
+      return *reinterpret_cast<Rec<TT, TV>*>(smem + (addr - base));
+#endif
     }
   };
 
-  // One 16-byte global -> LDS load (global_load_lds_dwordx4). lds_base is the
-  // wavefront-uniform LDS address of lane 0; lane i's record lands at
-  // lds_base + 16 i (lanes that do not call leave their slot untouched).
+  // One 16-byte global -> LDS load (global_load_lds_dwordx4). lds_addr is the
+  // wavefront-uniform LDS address of lane 0's 16 bytes; lane i's record lands
+  // at lds_addr + 16 i (lanes that do not call leave their slot untouched).
   // The builtin exists only in the device compilation.
-  __device__ __forceinline__ void lds_dma16(const void* src, unsigned char* lds_base)
+  __device__ __forceinline__ void lds_dma16(const void* src, uint32_t lds_addr)
   {
 #if __HIP_DEVICE_COMPILE__
-    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)lds_base, 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(uintptr_t)lds_addr, 16, 0, 0);
 #else
     (void)src;
-    (void)lds_base;
+    (void)lds_addr;
 #endif
   }
 
@@ -454,7 +467,7 @@ namespace sbamd
         if (j < n)
         {
           if constexpr (RB == 16)
-            lds_dma16(src + static_cast<size_t>(j) * RB, S.smem + dst + RB * static_cast<uint32_t>(c0));
+            lds_dma16(src + static_cast<size_t>(j) * RB, dst + RB * static_cast<uint32_t>(c0));
           else
             S.rec(dst + RB * static_cast<uint32_t>(j)) = reinterpret_cast<const RecT*>(src)[j];
         }
