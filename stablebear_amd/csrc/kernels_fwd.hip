@@ -98,20 +98,18 @@ namespace sbamd
       // Per pair: the two current records ordered by end time. The next union
       // breakpoint is always lo.t, and the side that owns lo advances (which
       // PCF lo belongs to does not matter to a symmetric pair op). lo_p /
-      // hi_p: LDS address of the owning PCF's next record. At the end of a
-      // phase both point at their slot's cursor, which becomes record 0 of the
-      // slot's next window, so between phases a pair only keeps which side
-      // owns lo. x is the first half of the pair op of the current segment
-      // (pair_pre); lo's value is not kept, since after a step the two current
-      // values are {new record's value, hi_v} whichever side advanced, and
-      // every pair op is symmetric in its operands. len is the length of the
-      // current segment [previous breakpoint, lo_t), formed in the time type
-      // when the segment starts and rounded to the value type.
+      // hi_p: LDS address of the owning PCF's next record; which window a
+      // pointer lies in tells its side (row windows precede column windows).
+      // x is the first half of the pair op of the current segment
+      // (pair_pre); lo's value is not kept, since after a step the two
+      // current values are {new record's value, hi_v} whichever side
+      // advanced, and every pair op is symmetric in its operands. len is the
+      // length of the current segment [previous breakpoint, lo_t), formed in
+      // the time type when the segment starts and rounded to the value type.
       TT lo_t[NCH], hi_t[NCH];
       TV len[NCH], x[NCH], hi_v[NCH];
       uint32_t lo_p[NCH], hi_p[NCH];
-      bool lo_g[NCH];
-      using Acc = typename std::conditional<KAHAN, KahanAcc, PlainAcc<TV>>::type;
+      using Acc = typename std::conditional<KAHAN, BlockKahanAcc, PlainAcc<TV>>::type;
       Acc acc[NCH];
 #pragma unroll
       for (int q = 0; q < NCH; ++q)
@@ -119,7 +117,6 @@ namespace sbamd
         lo_t[q] = hi_t[q] = L::inf();
         len[q] = x[q] = hi_v[q] = TV(0);
         lo_p[q] = hi_p[q] = 0;
-        lo_g[q] = false;
       }
       __syncthreads();
 
@@ -131,19 +128,22 @@ namespace sbamd
         phase_cursors(S, P, Tb, tid);
         __syncthreads();
 
-        // Steps of each pair in the phase: every record of either PCF that
-        // starts before Tb is consumed (the slots' cursor counts); in the
-        // first phase the two first records are read before the walk. The
-        // wavefront runs the smallest count over its lanes' pairs with both
-        // pairs' loads in flight, then the rest under the time compare.
-        uint32_t n_min = 0x7fffffffu;
-        uint32_t n_steps[NCH];
+        // Steps of each pair to reach the phase end: every record of either
+        // PCF that starts before Tb and is not yet consumed (the slots'
+        // cursor counts less the records the pair took past the previous
+        // phase end; a pair is either before Tb on both sides or past it on
+        // both); in the first phase the two first records are read before
+        // the walk. A pair may go on past Tb while the next record of its
+        // walk is staged, up to the earlier of its two slots' bounds, so
+        // lanes whose pairs reach Tb early keep walking while the wavefront
+        // waits for its slowest pair. Its budget counts the steps it can
+        // surely take: those to Tb and then the rest of the staged records of
+        // the side whose bound is earlier, each of which starts before that
+        // bound.
+        uint32_t n_need[NCH], n_cap[NCH];
 #pragma unroll
         for (int q = 0; q < NCH; ++q)
         {
-          const uint32_t n = static_cast<uint32_t>(S.cnt[slot_f[q]] + S.cnt[slot_g[q]] - (phase == 0 ? 2 : 0));
-          n_steps[q] = n;
-          n_min = n < n_min ? n : n_min;
           if (phase == 0)
           {
             const RecT rf = lds_rec(&S.rec(wb_f[q]));
@@ -157,11 +157,20 @@ namespace sbamd
             lo_p[q] = (k ? wb_f[q] : wb_g[q]) + RB;
             hi_p[q] = (k ? wb_g[q] : wb_f[q]) + RB;
           }
-          else
-          {
-            lo_p[q] = lo_g[q] ? wb_g[q] : wb_f[q];
-            hi_p[q] = lo_g[q] ? wb_f[q] : wb_g[q];
-          }
+          const bool lg = lo_p[q] >= wb_g[q];
+          const int of = static_cast<int>(((lg ? hi_p[q] : lo_p[q]) - wb_f[q]) / RB);
+          const int og = static_cast<int>(((lg ? lo_p[q] : hi_p[q]) - wb_g[q]) / RB);
+          const uint32_t tf = S.cnt[slot_f[q]], tg = S.cnt[slot_g[q]];
+          const int cf = static_cast<int>(tf & 0xffffu), cg = static_cast<int>(tg & 0xffffu);
+          const int n = cf + cg - of - og;
+          n_need[q] = n > 0 ? static_cast<uint32_t>(n) : 0u;
+          using Bits = typename L::Bits;
+          const bool fb = L::from_bits(static_cast<Bits>(S.bnd[slot_f[q]])) <=
+                          L::from_bits(static_cast<Bits>(S.bnd[slot_g[q]]));
+          const int ns_b = static_cast<int>((fb ? tf : tg) >> 16);
+          const int c_b = fb ? cf : cg;
+          const int o_b = fb ? of : og;
+          n_cap[q] = n_need[q] + static_cast<uint32_t>(ns_b - (o_b > c_b ? o_b : c_b));
         }
         // One merge step of pair q on the record r that lo_p[q] pointed at:
         // accumulate the segment ending at lo.t, then replace lo by r and
@@ -190,16 +199,43 @@ namespace sbamd
           for (int q = 0; q < NCH; ++q)
             step(q, r[q]);
         };
+        auto flush_all = [&]() {
+#pragma unroll
+          for (int q = 0; q < NCH; ++q)
+            acc[q].flush();
+        };
         auto step_one = [&](int q) { step(q, lds_rec(&S.rec(lo_p[q]))); };
 
-        // (1) every pair of every lane: scalar-counted, two steps per trip so
-        // that registers alternate roles instead of being copied
-        const int n_all = __builtin_amdgcn_readfirstlane(static_cast<int>(wave_min(n_min)));
+        // The wavefront walks until its slowest pair reaches Tb (the largest
+        // count), each lane at most its budget: (1) every pair of every lane
+        // up to the smallest of those limits over the wavefront; then (2)
+        // each pair up to the largest count, lanes past their budget masked
+        // off. Four steps per compensated add.
+        uint32_t n_top[NCH], c_min = 0x7fffffffu;
+#pragma unroll
+        for (int q = 0; q < NCH; ++q)
+        {
+          n_top[q] = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(wave_max(n_need[q]))));
+          c_min = n_cap[q] < c_min ? n_cap[q] : c_min;
+        }
+        int n_all = __builtin_amdgcn_readfirstlane(static_cast<int>(wave_min(c_min)));
+#pragma unroll
+        for (int q = 0; q < NCH; ++q)
+          n_all = n_all < static_cast<int>(n_top[q]) ? n_all : static_cast<int>(n_top[q]);
         int done = 0;
-        for (; done + 2 <= n_all; done += 2)
+        for (; done + 4 <= n_all; done += 4)
         {
           step_all();
           step_all();
+          step_all();
+          step_all();
+          flush_all();
+        }
+        if (done + 2 <= n_all)
+        {
+          step_all();
+          step_all();
+          done += 2;
         }
         if (done < n_all)
         {
@@ -209,38 +245,46 @@ namespace sbamd
 #pragma unroll
         for (int q = 0; q < NCH; ++q)
         {
-          // (2) pair q alone up to its own wavefront minimum, then (3) under
-          // the time compare until its next breakpoint reaches Tb
-          if constexpr (NCH > 1)
+          const int nt = static_cast<int>(n_top[q]);
+          // steps left to this lane in the phase
+          const int r = static_cast<int>(n_cap[q]) - done;
+          int d = 0;
+          for (; done + d + 4 <= nt; d += 4)
           {
-            const int nq = __builtin_amdgcn_readfirstlane(static_cast<int>(wave_min(n_steps[q])));
-            int d = done;
-            for (; d + 2 <= nq; d += 2)
-            {
+            if (d < r)
               step_one(q);
+            if (d + 1 < r)
               step_one(q);
-            }
-            if (d < nq)
+            if (d + 2 < r)
               step_one(q);
+            if (d + 3 < r)
+              step_one(q);
+            acc[q].flush();
           }
-          while (lo_t[q] < Tb)
-          {
-            step_one(q);
-            if (!(lo_t[q] < Tb))
-              break;
-            step_one(q);
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < NCH; ++q)
-        {
-          // row windows precede column windows
-          lo_g[q] = lo_p[q] >= wb_g[q];
+          for (; done + d < nt; ++d)
+            if (d < r)
+              step_one(q);
+          acc[q].flush();
           if (fold)
             acc[q].fold();
         }
+        const bool last = !(Tb < L::inf());
+        if (!last)
+        {
+          // the next windows start at this phase's cursors: move the
+          // pointers of pairs that went past Tb along with their records
+#pragma unroll
+          for (int q = 0; q < NCH; ++q)
+          {
+            const bool lg = lo_p[q] >= wb_g[q];
+            const uint32_t sf = (S.cnt[slot_f[q]] & 0xffffu) * RB;
+            const uint32_t sg = (S.cnt[slot_g[q]] & 0xffffu) * RB;
+            lo_p[q] -= lg ? sg : sf;
+            hi_p[q] -= lg ? sf : sg;
+          }
+        }
         __syncthreads();
-        if (!(Tb < L::inf()))
+        if (last)
           break;
       }
 

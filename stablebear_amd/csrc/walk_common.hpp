@@ -140,6 +140,7 @@ namespace sbamd
   {
     TV acc = TV(0);
     __device__ __forceinline__ void add(TV dt, TV w) { acc = fma(dt, w, acc); }
+    __device__ __forceinline__ void flush() {}
     __device__ __forceinline__ void fold() {}
     template <typename TT>
     __device__ __forceinline__ double finish(TT tail, TV w)
@@ -175,6 +176,47 @@ namespace sbamd
       // The final rectangle [t, MAX) is formed in the time type, whose MAX
       // bounds the domain (FLT_MAX or DBL_MAX), so a non-zero tail overflows
       // exactly as the reference's working type does; a zero tail adds 0.
+      const double r = big + (static_cast<double>(s) - static_cast<double>(c));
+      const TT term = tail * static_cast<TT>(w);
+      return r + static_cast<double>(term);
+    }
+  };
+
+  // Kahan-compensated FP32 sum over blocks of segments (the forward tile
+  // walk). add() chains a block's products into one FP32 partial with FMAs;
+  // flush() adds the partial to the compensated sum, which the walk does
+  // every 4 segments in its common steps and every 2 in its tails. A block of
+  // b non-negative terms carries a relative error of at most b u (u = 2^-24)
+  // and the compensated sum of the blocks at most 2u more, so an L1 / Lp sum
+  // stays within 6u = 3.6e-7 of the exact sum of its FP32 terms, at half the
+  // per-segment work of a compensated add per segment. fold() moves the sum
+  // into the FP64 accumulator (long pairs, every phase).
+  struct BlockKahanAcc
+  {
+    float s = 0.f;
+    float c = 0.f;
+    float part = 0.f;
+    double big = 0.0;
+    __device__ __forceinline__ void add(float dt, float w) { part = fmaf(dt, w, part); }
+    __device__ __forceinline__ void flush()
+    {
+      const float y = part - c;
+      const float t = s + y;
+      c = (t - s) - y;
+      s = t;
+      part = 0.f;
+    }
+    __device__ __forceinline__ void fold()
+    {
+      flush();
+      big += static_cast<double>(s) - static_cast<double>(c);
+      s = 0.f;
+      c = 0.f;
+    }
+    template <typename TT>
+    __device__ __forceinline__ double finish(TT tail, float w)
+    {
+      flush();
       const double r = big + (static_cast<double>(s) - static_cast<double>(c));
       const TT term = tail * static_cast<TT>(w);
       return r + static_cast<double>(term);
@@ -252,7 +294,8 @@ namespace sbamd
     int src_off;    // u64 per slot: global byte address of the next window's first record
     int tstart_off; // time type per slot: start time of the next window's first record
     int rem_off;    // i32 per slot: records from the next window's first to the PCF's end
-    int cnt_off;    // i32 per slot: records consumed in the current phase
+    int cnt_off;    // u32 per slot: records consumed in the current phase (low 16 bits) and records staged (high 16)
+    int bnd_off;    // time type (bits, 8 bytes) per slot: start of the first record not staged
     int tb_off;     // phase end time (bits)
     int misc_off;   // next tile of a persistent workgroup
     int rows_off;   // R FP64 row-sum partials
@@ -277,6 +320,8 @@ namespace sbamd
     L.cnt_off = o;
     o += 4 * NS;
     o = align_up(o, 8);
+    L.bnd_off = o;
+    o += 8 * NS;
     L.tb_off = o;
     o += 8;
     L.misc_off = o;
@@ -296,7 +341,8 @@ namespace sbamd
     unsigned long long* src;
     TT* tstart;
     int32_t* rem;
-    int32_t* cnt;
+    uint32_t* cnt;
+    unsigned long long* bnd;
     typename Lim<TT>::Bits* tb;
 
     __device__ TileLds(unsigned char* s, const LdsLayout& L) : smem(s), lay(L)
@@ -309,7 +355,8 @@ namespace sbamd
       src = reinterpret_cast<unsigned long long*>(s + L.src_off);
       tstart = reinterpret_cast<TT*>(s + L.tstart_off);
       rem = reinterpret_cast<int32_t*>(s + L.rem_off);
-      cnt = reinterpret_cast<int32_t*>(s + L.cnt_off);
+      cnt = reinterpret_cast<uint32_t*>(s + L.cnt_off);
+      bnd = reinterpret_cast<unsigned long long*>(s + L.bnd_off);
       tb = reinterpret_cast<typename Lim<TT>::Bits*>(s + L.tb_off);
     }
 
@@ -370,6 +417,20 @@ namespace sbamd
     const uint32_t c = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 47));
     const uint32_t d = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
     return min(min(a, b), min(c, d));
+  }
+
+  // Maximum over the 64 lanes of a wavefront, as wave_min.
+  __device__ __forceinline__ uint32_t wave_max(uint32_t v)
+  {
+    v = max(v, dpp_shr<1>(v));
+    v = max(v, dpp_shr<2>(v));
+    v = max(v, dpp_shr<4>(v));
+    v = max(v, dpp_shr<8>(v));
+    const uint32_t a = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 15));
+    const uint32_t b = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 31));
+    const uint32_t c = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 47));
+    const uint32_t d = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+    return max(max(a, b), max(c, d));
   }
 
   template <int SHIFT>
@@ -437,7 +498,9 @@ namespace sbamd
   // vmcnt wait covers them all. Wavefront 0 also finds the phase end: each
   // slot with more than cap records left proposes the end time of the last
   // record it stages, and the phase ends at the earliest proposal (+inf
-  // when every slot stages all it has left).
+  // when every slot stages all it has left). Each slot's proposal is kept as
+  // its bound: a pair may walk on past the phase end while the next record
+  // of either side starts before the earlier of its two slots' bounds.
   template <typename TT, typename TV>
   __device__ void phase_stage(const TileLds<TT, TV>& S, const FwdParams& P, int tid, int nthreads)
   {
@@ -476,6 +539,8 @@ namespace sbamd
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     if (w == 0)
     {
+      if (lane < NS)
+        S.bnd[lane] = static_cast<unsigned long long>(prop);
       const typename L::Bits tb = wave_min(prop);
       if (lane == 0)
         *S.tb = tb;
@@ -484,9 +549,10 @@ namespace sbamd
 
   // Second part (after the barrier): the cursor of every slot -- the number
   // of staged records that start before the phase end Tb, found by a binary
-  // search over the window (starts increase along it) -- and the slot's next
-  // window: its first record, start time and remaining count. One lane of
-  // wavefront 0 per slot, all slots at once.
+  // search over the window (starts increase along it), kept with the number
+  // of staged records -- and the slot's next window: its first record, start
+  // time and remaining count. One lane of wavefront 0 per slot, all slots at
+  // once.
   template <typename TT, typename TV>
   __device__ void phase_cursors(const TileLds<TT, TV>& S, const FwdParams& P, TT Tb, int tid)
   {
@@ -511,7 +577,7 @@ namespace sbamd
         else
           hi = mid;
       }
-      S.cnt[tid] = lo;
+      S.cnt[tid] = static_cast<uint32_t>(lo) | (static_cast<uint32_t>(n) << 16);
       if (lo > 0)
       {
         S.tstart[tid] = S.rec(wb + RB * static_cast<uint32_t>(lo - 1)).t;
