@@ -245,7 +245,10 @@ int sbamd_plain_destroy(sbamd_plain* plain);
  * (no reference counterpart). op must be SBAMD_OP_LP with p == 2 (fwd_out =
  * the forward distances) or SBAMD_OP_DOT. grad_v / grad_t are device arrays
  * indexed like the set's original points (FP32 or FP64 per grad_dtype),
- * overwritten. Single-set plans only. */
+ * overwritten. Single-set plans only. The first backward of a plan sorts
+ * the set's breakpoints into the time-ordered event lists the kernel walks
+ * and keeps them with the plan's working arrays; later backwards of the plan
+ * (a set's times never change) only refresh the lists' value jumps. */
 int sbamd_plan_backward(sbamd_plan* plan, const void* grad_out_device, const void* fwd_out_device,
                         int32_t out_dtype, void* grad_v_device, void* grad_t_device,
                         int32_t grad_dtype, void* stream);

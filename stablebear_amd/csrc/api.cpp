@@ -759,7 +759,7 @@ namespace sbamd
     }
 
     // The backward launches on working arrays `wk` (no allocation here).
-    void run_backward(const Plan& pl, int kop, const BwdWork& wk, const void* grad_out_device,
+    void run_backward(const Plan& pl, int kop, const BwdWork& wk, bool lists_ready, const void* grad_out_device,
                       const void* fwd_out_device, int32_t out_dtype, void* grad_v_device,
                       void* grad_t_device, int32_t grad_dtype, hipStream_t st)
     {
@@ -809,7 +809,7 @@ namespace sbamd
       B.wsum = wk.w;
       B.grad_v = grad_v_device;
       B.grad_t = grad_t_device;
-      launch_backward(B, st);
+      launch_backward(B, lists_ready, st);
     }
 
     // Record offsets of a single-set plan's PCFs in the original order, for
@@ -2186,6 +2186,7 @@ extern "C"
         plan->bwd_q = DeviceBuffer(M * 8);
         plan->bwd_a = DeviceBuffer(M * 8);
         plan->bwd_w = DeviceBuffer(N * 8);
+        plan->bwd_lists_ready = false;
       }
       BwdWork wk;
       wk.key0 = plan->bwd_key0.as<double>();
@@ -2202,8 +2203,9 @@ extern "C"
       wk.q = plan->bwd_q.as<double>();
       wk.a = plan->bwd_a.as<double>();
       wk.w = plan->bwd_w.as<double>();
-      run_backward(*plan, kop, wk, grad_out_device, fwd_out_device, out_dtype, grad_v_device, grad_t_device,
-                   grad_dtype, pick_stream(plan->ctx, stream));
+      run_backward(*plan, kop, wk, plan->bwd_lists_ready, grad_out_device, fwd_out_device, out_dtype,
+                   grad_v_device, grad_t_device, grad_dtype, pick_stream(plan->ctx, stream));
+      plan->bwd_lists_ready = true;
     });
   }
 
@@ -2239,8 +2241,9 @@ extern "C"
       wk.ids[0] = reinterpret_cast<int32_t*>(base + NOALLOC_IDS0);
       wk.ids[1] = reinterpret_cast<int32_t*>(base + NOALLOC_IDS1);
       wk.w = reinterpret_cast<double*>(base + NOALLOC_W);
-      run_backward(*plan, kop, wk, grad_out_device, fwd_out_device, out_dtype, grad_v_device, grad_t_device,
-                   grad_dtype, pick_stream(plan->ctx, stream));
+      // the working array is shared by every plan on the device: sort every time
+      run_backward(*plan, kop, wk, false, grad_out_device, fwd_out_device, out_dtype, grad_v_device,
+                   grad_t_device, grad_dtype, pick_stream(plan->ctx, stream));
     });
   }
 

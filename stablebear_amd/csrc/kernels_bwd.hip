@@ -398,6 +398,25 @@ namespace sbamd
     out[p] = ev;
   }
 
+  // Thread per event of both lists (the bands' lists, then the sublists):
+  // the value jump at the event and its product with the time, from the
+  // current values. The lists' order, times and records depend on the
+  // breakpoint times only, so a backward over a set whose lists were already
+  // sorted refreshes these two fields instead of sorting again; the results
+  // are those of bwd_events_kernel bit for bit.
+  __global__ void bwd_refresh_kernel(BwdParams B)
+  {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= 2 * B.records)
+      return;
+    BwdEvent* const ev = e < B.records ? B.events + e : B.events2 + (e - B.records);
+    const int32_t id = ev->pos;
+    const bool first = (ev->col & BWD_FIRST) != 0;
+    const double d = B.val0[id] - (first ? 0.0 : B.val0[id - 1]);
+    ev->d = d;
+    ev->dt = d * ev->t;
+  }
+
   // Wavefront per PCF: pst0[p] = F(t_start(p)) - v_p t_start(p), with F the
   // PCF's integral from 0 (an inclusive scan of the record integrals over
   // the lanes, 64 records at a time).
@@ -752,7 +771,7 @@ namespace sbamd
     }
   }
 
-  void launch_backward(const BwdParams& prm, hipStream_t stream)
+  void launch_backward(const BwdParams& prm, bool lists_ready, hipStream_t stream)
   {
     if (prm.records <= 0 || prm.n <= 0)
       return;
@@ -767,16 +786,22 @@ namespace sbamd
     int from = 0;
     for (int32_t width = 1; width < BWD_GROUP; width *= 2)
     {
-      if (width == vw)
+      if (width == vw && !lists_ready)
       {
         hipLaunchKernelGGL(bwd_events_kernel, dim3(rgrid), dim3(256), 0, stream, prm, from, vw, prm.events2);
         SBAMD_HIP(hipGetLastError());
       }
-      hipLaunchKernelGGL(bwd_merge_kernel, dim3(rgrid), dim3(256), 0, stream, prm, width, from, BWD_GROUP);
-      SBAMD_HIP(hipGetLastError());
+      if (!lists_ready)
+      {
+        hipLaunchKernelGGL(bwd_merge_kernel, dim3(rgrid), dim3(256), 0, stream, prm, width, from, BWD_GROUP);
+        SBAMD_HIP(hipGetLastError());
+      }
       from ^= 1;
     }
-    hipLaunchKernelGGL(bwd_events_kernel, dim3(rgrid), dim3(256), 0, stream, prm, from, BWD_GROUP, prm.events);
+    if (lists_ready)
+      hipLaunchKernelGGL(bwd_refresh_kernel, dim3(2 * rgrid), dim3(256), 0, stream, prm);
+    else
+      hipLaunchKernelGGL(bwd_events_kernel, dim3(rgrid), dim3(256), 0, stream, prm, from, BWD_GROUP, prm.events);
     SBAMD_HIP(hipGetLastError());
     // the sort keys are free now: pst0 takes the other buffer
     BwdParams B = prm;

@@ -311,6 +311,9 @@ namespace sbamd
     // backward working arrays (kernels_bwd.hip), allocated on first use
     DeviceBuffer bwd_ooff, bwd_key0, bwd_val0, bwd_col0, bwd_src0, bwd_keys[2], bwd_ids[2],
         bwd_events, bwd_events2, bwd_q, bwd_a, bwd_w;
+    // the event lists in bwd_events / bwd_events2 are sorted (a set's
+    // breakpoint times never change, so they stay sorted until reallocated)
+    bool bwd_lists_ready = false;
     // long-pair kernel: mode 0 off, 1 on, 2 auto (few long pairs, FP32C);
     // use_long is the decision for this plan
     int32_t long_mode = 2;
@@ -331,7 +334,9 @@ namespace sbamd
   void launch_long(const LongParams& prm, int tt_f64, int tv_f64, int op, int kahan, int cu_count,
                    hipStream_t stream);
   void launch_norm(const NormParams& prm, int tt_f64, int tv_f64, hipStream_t stream);
-  void launch_backward(const BwdParams& prm, hipStream_t stream);
+  // lists_ready: the event lists of this set were sorted by an earlier
+  // backward on the same working arrays (only their value fields are redone)
+  void launch_backward(const BwdParams& prm, bool lists_ready, hipStream_t stream);
   void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream);
   void launch_gather_subset(const int64_t* roff, const void* rec, int rec_bytes, const int64_t* sel,
                             const int64_t* droff, int64_t n, int64_t records, void* drec, hipStream_t stream);

@@ -202,3 +202,46 @@ def test_cached_set_keeps_each_forwards_values(gpu):
     m = _interior_times(off)
     et = et_a + et_b
     assert np.max(np.abs(tt.grad.cpu().numpy()[m] - et[m])) <= 1e-9 * np.abs(et).max()
+
+
+@pytest.mark.parametrize("op", ["lp", "dot"])
+def test_repeated_backward_reuses_sorted_lists(gpu, op):
+    """A plan's first backward sorts the set's breakpoints into event lists;
+    later backwards keep that order and refresh only the value jumps. After
+    the set's values change on the device, the second backward of the same
+    plan matches a backward over a new set with those values (sorted
+    afresh) and the oracle's analytic gradient."""
+    from stablebear_amd import runtime
+
+    off, t, v, w = _inputs(500, seed=41)
+    v2 = np.random.default_rng(42).standard_normal(len(v))
+    v2[off[1:] - 1] = 0.0
+    s = torch.cuda.current_stream().cuda_stream
+    p = 2.0 if op == "lp" else 1.0
+
+    def run(plan, npts):
+        out = torch.empty(plan.out_count, dtype=torch.float64, device="cuda")
+        plan.execute(out.data_ptr(), True, stream=s)
+        g = torch.from_numpy(np.random.default_rng(43).uniform(0, 1, plan.out_count)).cuda()
+        gv = torch.full((npts,), np.nan, dtype=torch.float64, device="cuda")
+        gt = torch.full((npts,), np.nan, dtype=torch.float64, device="cuda")
+        plan.backward(g.data_ptr(), out.data_ptr(), True, gv.data_ptr(), gt.data_ptr(), True, stream=s)
+        torch.cuda.synchronize()
+        return g.cpu().numpy(), out.cpu().numpy(), gv.cpu().numpy(), gt.cpu().numpy()
+
+    ds = runtime.DeviceSet(runtime.csr_from_arrays(off, t, v), "fp64")
+    plan = runtime.Plan(ds, op=op, p=p)
+    run(plan, len(v))  # sorts the lists
+    v2_d = torch.tensor(v2, dtype=torch.float64, device="cuda")
+    ds.update_values(v2_d.data_ptr(), s)
+    g, fwd, gv, gt = run(plan, len(v))  # reuses them
+    ds_new = runtime.DeviceSet(runtime.csr_from_arrays(off, t, v2), "fp64")
+    _, _, gv_new, gt_new = run(runtime.Plan(ds_new, op=op, p=p), len(v))
+    assert np.max(np.abs(gv - gv_new)) <= 1e-12 * np.abs(gv_new).max()
+    assert np.max(np.abs(gt - gt_new)) <= 1e-12 * np.abs(gt_new).max()
+    oop = oracle.OP_LP if op == "lp" else oracle.OP_DOT
+    okind = oracle.KIND_PDIST if op == "lp" else oracle.KIND_KERNEL
+    ev, et = oracle.grad(oop, okind, off, t, v2, g, fwd if op == "lp" else None)
+    assert np.max(np.abs(gv - ev)) <= 1e-9 * np.abs(ev).max()
+    m = _interior_times(off)
+    assert np.max(np.abs(gt[m] - et[m])) <= 1e-9 * np.abs(et).max()
