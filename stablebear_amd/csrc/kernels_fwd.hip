@@ -8,7 +8,11 @@
 // in LDS in time phases: every staged record serves all R (or C) pairs that
 // use it. A phase ends at the time Tb where the first tile PCF would overflow
 // `cap` staged records; each lane walks its pairs up to Tb, then the next
-// records are staged.
+// records are staged. A wavefront waits for its slowest pair to reach Tb;
+// meanwhile lanes whose pairs got there first walk on into the records that
+// are already staged past Tb (their budget), so the next phase has less left
+// for them, and the next windows start at the phase's cursors with the
+// pointers of such pairs moved along.
 //
 // The walk step is branch-free: one compare picks the side whose record ends
 // first, one LDS load fetches that side's next record, and selects update the
