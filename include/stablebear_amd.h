@@ -201,9 +201,11 @@ int sbamd_plan_out_offset(const sbamd_plan* plan, int32_t problem, uint64_t* off
  * partials into FP64 per phase (default 8192). */
 int sbamd_plan_set_fold_threshold(sbamd_plan* plan, int64_t segments);
 /* Persistent launch (on by default for batched plans): as many workgroups as
- * fit on the device take tiles, longest first, from an atomic counter that a
- * fill kernel resets before each launch (graph-capture safe). Off: one
- * workgroup per tile. */
+ * fit on the device take tiles from atomic counters that a fill kernel resets
+ * before each launch (graph-capture safe). A batched plan keeps a work queue
+ * per XCD (units of tiles that share their PCFs, dealt largest first to the
+ * least-loaded queue, longest tiles first within a unit); a workgroup takes
+ * from its XCD's queue, then from the others. Off: one workgroup per tile. */
 int sbamd_plan_set_persistent(sbamd_plan* plan, int32_t on);
 /* Long-pair kernel: one wavefront per pair, the pair's union segments split
  * across the 64 lanes at merge-path points, partial sums added in FP64. For
@@ -424,6 +426,14 @@ int sbamd_count_segments(const sbamd_pcf_set* rows, const sbamd_pcf_set* cols,
 int sbamd_plan_tiles_host(int64_t n_rows, const int64_t* m_rows, int64_t n_cols,
                           const int64_t* m_cols, int32_t tile_rows, int32_t tile_cols,
                           int32_t* tiles, int64_t cap, int64_t* n_tiles, int64_t* perm_rows);
+/* The per-XCD work queues a batched (persistent) plan gives a tile list (no
+ * reference counterpart; a planner check): tiles are (row0, col0, problem)
+ * int32 triples of tile_rows x tile_cols tiles, in longest-first order, with
+ * their work; n_queues queues. order receives the new position of every
+ * input tile's index (the reordered list is tiles[order[0]], tiles[order[1]],
+ * ...), queue_off the n_queues + 1 queue starts in it. */
+int sbamd_queue_order_host(int64_t n_tiles, const int32_t* tiles, const int64_t* work, int32_t tile_rows,
+                           int32_t tile_cols, int32_t n_queues, int64_t* order, int64_t* queue_off);
 
 #ifdef __cplusplus
 }

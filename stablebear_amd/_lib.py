@@ -159,6 +159,8 @@ _SIGS = {
     "sbamd_plan_tiles_host": (None, [ctypes.c_int64, c_i64p, ctypes.c_int64, c_i64p,
                                      ctypes.c_int32, ctypes.c_int32, c_i32p, ctypes.c_int64,
                                      c_i64p, c_i64p]),
+    "sbamd_queue_order_host": (None, [ctypes.c_int64, c_i32p, c_i64p, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.c_int32, c_i64p, c_i64p]),
 }
 
 EXPORTED = tuple(_SIGS)

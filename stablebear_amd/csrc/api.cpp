@@ -14,10 +14,12 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <new>
 #include <thread>
+#include <tuple>
 #include <type_traits>
 #include <unordered_map>
 
@@ -553,8 +555,17 @@ namespace sbamd
       use_device(pl.ctx);
       upload(pl.probs, pl.h_probs);
       upload(pl.tiles, pl.h_tiles);
+      std::vector<int64_t> qtab(NCLASS * 2 * DEVICE_QUEUES);
+      for (int k = 0; k < NCLASS; ++k)
+        for (int x = 0; x < DEVICE_QUEUES; ++x)
+        {
+          qtab[k * 2 * DEVICE_QUEUES + x] = pl.xq[k][x];
+          qtab[k * 2 * DEVICE_QUEUES + DEVICE_QUEUES + x] = pl.xq[k][x + 1] - pl.xq[k][x];
+        }
+      upload(pl.qtab, qtab);
       if (!pl.counter.ptr)
-        pl.counter = DeviceBuffer(64 * NCLASS); // one tile counter per class launch, 64 bytes apart
+        // one tile counter per queue and class launch, 64 bytes apart
+        pl.counter = DeviceBuffer(64 * NCLASS * DEVICE_QUEUES);
     }
 
     FwdParams fwd_params(const Plan& pl, void* out, int32_t out_dtype, double* row_sum)
@@ -693,7 +704,12 @@ namespace sbamd
         P.tiles = pl.tiles.as<Tile>() + a;
         P.ntiles = b - a;
         if (pl.persistent)
-          P.work_counter = reinterpret_cast<uint32_t*>(pl.counter.as<unsigned char>() + 64 * k);
+        {
+          P.work_counter = reinterpret_cast<uint32_t*>(pl.counter.as<unsigned char>() + 64 * DEVICE_QUEUES * k);
+          // the class's queues when the whole class runs, else one queue
+          const bool whole = a == pl.class_begin[k] && b == pl.class_begin[k + 1];
+          P.q_tab = whole ? pl.qtab.as<int64_t>() + 2 * DEVICE_QUEUES * k : nullptr;
+        }
         launch_forward(P, pl.tt_f64, pl.tv_f64, kernel_op(pl.op, pl.p),
                        pl.precision == SBAMD_PREC_FP32C ? 1 : 0, stream);
       }
@@ -1113,9 +1129,11 @@ namespace sbamd
     h_tiles.clear();
     const bool diag = out_kind == SBAMD_OUT_SYMMAT;
     // Longest tiles first over [b0, b1) of h_tiles (LPT order for a launch
-    // whose tiles come from several problems or regions).
+    // whose tiles come from several problems or regions); returns each
+    // tile's work (pairs x longest pair) in the new order.
     auto lpt = [&](size_t b0, size_t b1) {
       std::vector<std::pair<int64_t, size_t>> cost(b1 - b0);
+      std::vector<int64_t> work_of(b1 - b0);
       for (size_t k = b0; k < b1; ++k)
       {
         const Tile& t = h_tiles[k];
@@ -1127,12 +1145,27 @@ namespace sbamd
         for (int q = 0; q < t.nc; ++q)
           mc = std::max(mc, cs.h_roff[t.c0 + q + 1] - cs.h_roff[t.c0 + q]);
         cost[k - b0] = {-(mr + mc), k};
+        work_of[k - b0] = static_cast<int64_t>(t.nr) * t.nc * (mr + mc);
       }
       std::stable_sort(cost.begin(), cost.end());
       std::vector<Tile> sorted(b1 - b0);
+      std::vector<int64_t> work(b1 - b0);
       for (size_t k = 0; k < cost.size(); ++k)
+      {
         sorted[k] = h_tiles[cost[k].second];
+        work[k] = work_of[cost[k].second - b0];
+      }
       std::copy(sorted.begin(), sorted.end(), h_tiles.begin() + static_cast<int64_t>(b0));
+      return work;
+    };
+    // class k over [b0, b1): per-XCD queues for a batched (persistent) plan,
+    // else every tile in queue 0
+    auto queues = [&](int k, size_t b0, size_t b1, int qr, int qc, const std::vector<int64_t>* work) {
+      if (work)
+        xcd_queue_order(h_tiles, b0, b1, qr, qc, DEVICE_QUEUES, *work, xq[k]);
+      else
+        for (int x = 0; x <= DEVICE_QUEUES; ++x)
+          xq[k][x] = x == 0 ? 0 : static_cast<int64_t>(b1 - b0);
     };
     // sorted positions where each length class starts
     auto class_bounds = [](const std::vector<int64_t>& m, int64_t* at) {
@@ -1155,12 +1188,20 @@ namespace sbamd
           build_tiles(m, nullptr, R, C, diag, static_cast<uint16_t>(b), fold_threshold, h_tiles);
         }
       if (batched)
-        lpt(0, h_tiles.size());
+      {
+        const std::vector<int64_t> work = lpt(0, h_tiles.size());
+        queues(0, 0, h_tiles.size(), R, C, &work);
+      }
       else
+      {
         xcd_supertile_order(h_tiles, 0, h_tiles.size(), R, C, DEVICE_XCDS);
+        queues(0, 0, h_tiles.size(), R, C, nullptr);
+      }
       class_begin[0] = 0;
       for (int k = 1; k <= NCLASS; ++k)
         class_begin[k] = static_cast<int64_t>(h_tiles.size());
+      for (int k = 1; k < NCLASS; ++k)
+        queues(k, 0, 0, R, C, nullptr);
     }
     else
     {
@@ -1199,9 +1240,10 @@ namespace sbamd
             build_tiles_region(ms[b], ms[b], a[k], a[k + 1], a[k], a[k + 1], sh.R, sh.R, true, diag, prob,
                                fold_threshold, h_tiles);
           }
-        lpt(b0, h_tiles.size());
+        const std::vector<int64_t> work = lpt(b0, h_tiles.size());
         if (!batched)
           xcd_supertile_order(h_tiles, b0, h_tiles.size(), sh.R, sh.C, DEVICE_XCDS);
+        queues(k, b0, h_tiles.size(), sh.R, sh.C, batched ? &work : nullptr);
         class_begin[k + 1] = static_cast<int64_t>(h_tiles.size());
       }
     }
@@ -2789,6 +2831,42 @@ extern "C"
         *segments = n > 0 ? n * M - (n > 1 ? tri(n) : 0) : 0;
       else
         *segments = n > 1 ? (n - 1) * M - tri(n) : 0;
+    });
+  }
+
+  int sbamd_queue_order_host(int64_t n_tiles, const int32_t* tiles, const int64_t* work, int32_t tile_rows,
+                             int32_t tile_cols, int32_t n_queues, int64_t* order, int64_t* queue_off)
+  {
+    return guarded([&] {
+      need(order, "order");
+      need(queue_off, "queue_off");
+      if (n_tiles < 0 || (n_tiles > 0 && (!tiles || !work)))
+        fail(SBAMD_ERR_INVALID, "bad tile list");
+      if (n_queues < 1 || n_queues > 64 || tile_rows < 1 || tile_cols < 1)
+        fail(SBAMD_ERR_INVALID, "bad queue count or tile shape");
+      // the tiles carry their input index in c0's place through the reorder
+      std::vector<Tile> t(static_cast<size_t>(n_tiles));
+      std::vector<int64_t> w(work, work + n_tiles);
+      for (int64_t k = 0; k < n_tiles; ++k)
+      {
+        t[k].r0 = tiles[3 * k + 0];
+        t[k].c0 = tiles[3 * k + 1];
+        t[k].prob = static_cast<uint16_t>(tiles[3 * k + 2]);
+        t[k].nr = static_cast<int16_t>(tile_rows);
+        t[k].nc = static_cast<int16_t>(tile_cols);
+        t[k].flags = 0;
+      }
+      std::vector<Tile> moved(t);
+      std::vector<int64_t> qoff(static_cast<size_t>(n_queues) + 1);
+      xcd_queue_order(moved, 0, moved.size(), tile_rows, tile_cols, n_queues, w, qoff.data());
+      // recover each moved tile's input index: match (prob, r0, c0), unique per tile
+      std::map<std::tuple<int, int, int>, int64_t> where;
+      for (int64_t k = 0; k < n_tiles; ++k)
+        where[std::make_tuple(static_cast<int>(t[k].prob), t[k].r0, t[k].c0)] = k;
+      for (int64_t k = 0; k < n_tiles; ++k)
+        order[k] = where.at(std::make_tuple(static_cast<int>(moved[k].prob), moved[k].r0, moved[k].c0));
+      for (int x = 0; x <= n_queues; ++x)
+        queue_off[x] = qoff[x];
     });
   }
 

@@ -50,9 +50,12 @@ namespace sbamd
     const int half = tile_lane_cols(P.C, NCH);
 
     // Tiles come either one per workgroup (grid = tiles, the dispatcher hands
-    // them out in launch order) or, for persistent workgroups, from an atomic
-    // counter over the longest-first tile list.
+    // them out in launch order) or, for persistent workgroups, from per-XCD
+    // queues: the dispatcher places workgroup w on XCD w mod 8, which takes
+    // tiles from its own queue (tiles that share their PCFs, so they stay in
+    // that XCD's L2) through an atomic counter, then from the other queues.
     uint32_t* const s_next_tile = reinterpret_cast<uint32_t*>(smem + lay.misc_off);
+    uint32_t queue = blockIdx.x & (DEVICE_QUEUES - 1);
     // row sums (when requested): the tile's R partial sums, one global atomic
     // per row and tile instead of one per pair
     double* const s_rows = reinterpret_cast<double*>(smem + lay.rows_off);
@@ -62,9 +65,25 @@ namespace sbamd
       if (P.work_counter)
       {
         if (tid == 0)
-          *s_next_tile = atomicAdd(P.work_counter, 1u);
+        {
+          uint32_t next = 0xffffffffu; // every queue is empty
+          for (int tries = 0; tries < DEVICE_QUEUES; ++tries)
+          {
+            const int64_t qb = P.q_tab ? P.q_tab[queue] : 0;
+            const int64_t ql = P.q_tab ? P.q_tab[DEVICE_QUEUES + queue] : (queue == 0 ? P.ntiles : 0);
+            const uint32_t t = atomicAdd(P.work_counter + 16 * queue, 1u);
+            if (static_cast<int64_t>(t) < ql)
+            {
+              next = static_cast<uint32_t>(qb) + t;
+              break;
+            }
+            queue = (queue + 1) & (DEVICE_QUEUES - 1);
+          }
+          *s_next_tile = next;
+        }
         __syncthreads();
-        ti = *s_next_tile;
+        const uint32_t nt = *s_next_tile;
+        ti = nt == 0xffffffffu ? P.ntiles : static_cast<int64_t>(nt);
       }
       if (ti >= P.ntiles)
         break;

@@ -374,4 +374,79 @@ namespace sbamd
     return 0;
   }
 
+  void xcd_queue_order(std::vector<Tile>& tiles, size_t b0, size_t b1, int R, int C, int nxcd,
+                       const std::vector<int64_t>& work, int64_t* qoff)
+  {
+    const size_t n = b1 - b0;
+    const size_t nx = static_cast<size_t>(nxcd);
+    for (size_t x = 0; x <= nx; ++x)
+      qoff[x] = x == 0 ? 0 : static_cast<int64_t>(n);
+    if (nxcd <= 1 || n == 0)
+      return;
+    // units: G x G tiles of one problem (they share G R rows and G C
+    // columns), G the largest that leaves at least 8 units per queue
+    std::vector<uint32_t> uid(n);
+    size_t nu = 0;
+    for (int G : {16, 8, 4, 2, 1})
+    {
+      const int64_t sr = static_cast<int64_t>(G) * R, sc = static_cast<int64_t>(G) * C;
+      std::unordered_map<uint64_t, uint32_t> id;
+      for (size_t k = 0; k < n; ++k)
+      {
+        const Tile& t = tiles[b0 + k];
+        const uint64_t key = (static_cast<uint64_t>(t.prob) << 48) |
+                             (static_cast<uint64_t>(t.r0 / sr) << 24) | static_cast<uint64_t>(t.c0 / sc);
+        uid[k] = id.emplace(key, static_cast<uint32_t>(id.size())).first->second;
+      }
+      nu = id.size();
+      if (nu >= 8 * nx || G == 1)
+        break;
+    }
+    // unit work and its first tile in the (longest-first) list
+    std::vector<int64_t> uwork(nu, 0);
+    std::vector<size_t> ufirst(nu, n);
+    for (size_t k = 0; k < n; ++k)
+    {
+      uwork[uid[k]] += work[k];
+      ufirst[uid[k]] = std::min(ufirst[uid[k]], k);
+    }
+    // units by decreasing work, each to the queue with the least work so far
+    std::vector<uint32_t> order(nu);
+    for (size_t u = 0; u < nu; ++u)
+      order[u] = static_cast<uint32_t>(u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+      return uwork[a] != uwork[b] ? uwork[a] > uwork[b] : ufirst[a] < ufirst[b];
+    });
+    std::vector<int64_t> load(nx, 0);
+    std::vector<uint32_t> uq(nu), urank(nu);
+    std::vector<std::vector<uint32_t>> qunits(nx);
+    for (uint32_t u : order)
+    {
+      const size_t x = static_cast<size_t>(std::min_element(load.begin(), load.end()) - load.begin());
+      load[x] += uwork[u];
+      uq[u] = static_cast<uint32_t>(x);
+      urank[u] = static_cast<uint32_t>(qunits[x].size());
+      qunits[x].push_back(u);
+    }
+    // layout: queue 0's units, then queue 1's, ...; a unit's tiles stay in list order
+    std::vector<size_t> ucount(nu, 0), ustart(nu, 0);
+    for (size_t k = 0; k < n; ++k)
+      ++ucount[uid[k]];
+    size_t at = 0;
+    for (size_t x = 0; x < nx; ++x)
+    {
+      qoff[x] = static_cast<int64_t>(at);
+      for (uint32_t u : qunits[x])
+      {
+        ustart[u] = at;
+        at += ucount[u];
+      }
+    }
+    qoff[nx] = static_cast<int64_t>(at);
+    std::unique_ptr<Tile[]> out(new Tile[n]);
+    for (size_t k = 0; k < n; ++k)
+      out[ustart[uid[k]]++] = tiles[b0 + k];
+    std::copy(out.get(), out.get() + n, tiles.begin() + static_cast<std::ptrdiff_t>(b0));
+  }
+
 }

@@ -79,6 +79,9 @@ namespace sbamd
   };
 
   // Kernel parameters of the forward lane-per-pair tile kernel.
+  // Work queues of a persistent launch, one per XCD.
+  constexpr int DEVICE_QUEUES = 8;
+
   struct FwdParams
   {
     const int64_t* r_off;  // record offsets of the row set (sorted PCF order), n+1 per problem base
@@ -100,7 +103,11 @@ namespace sbamd
     int32_t ppl; // pairs per lane (1 or 2)
     int32_t symmetric; // rows and cols are the same set (pdist / kernel)
     int32_t include_diag;
-    uint32_t* work_counter; // non-null: persistent workgroups take tiles from this counter
+    uint32_t* work_counter; // non-null: persistent workgroups take tiles from these counters (64 bytes apart)
+    // persistent: queue x holds tiles q_tab[x] .. q_tab[x] + q_tab[8 + x] - 1;
+    // a workgroup takes from the queue of its XCD, then from the others (null:
+    // every tile in queue 0)
+    const int64_t* q_tab;
   };
 
   // Kernel parameters of the long-pair kernel (one wavefront per pair).
@@ -293,6 +300,11 @@ namespace sbamd
     bool per_class = false;
     Shape cls[NCLASS] = {};
     int64_t class_begin[NCLASS + 1] = {};
+    // persistent launches: class k's tiles in per-XCD queues, queue x at
+    // class_begin[k] + xq[k][x] .. class_begin[k] + xq[k][x + 1] (all in
+    // queue 0 when the tiles were not ordered so)
+    int64_t xq[NCLASS][DEVICE_QUEUES + 1] = {};
+    DeviceBuffer qtab; // per class: the queues' starts, then their lengths (FwdParams::q_tab)
     Shape shape_of(int k) const { return per_class ? cls[k] : Shape{R, C, cap, ppl}; }
     int64_t fold_threshold = 8192;
     uint64_t pairs = 0, segments = 0, out_count = 0;
@@ -413,6 +425,13 @@ namespace sbamd
                    std::vector<Tile>& tiles);
   void sort_by_records(const std::vector<int64_t>& m, std::vector<int64_t>& perm);
   int xcd_supertile_order(std::vector<Tile>& tiles, size_t b0, size_t b1, int R, int C, int nxcd);
+  // Per-XCD work queues of a persistent launch over tiles [b0, b1) (given in
+  // longest-first order, `work` per tile): units of G x G tiles of one
+  // problem, which share their PCFs, are dealt largest first to the queue
+  // with the least work so far; the queues are laid out one after the other,
+  // queue x at [b0 + qoff[x], b0 + qoff[x + 1]).
+  void xcd_queue_order(std::vector<Tile>& tiles, size_t b0, size_t b1, int R, int C, int nxcd,
+                       const std::vector<int64_t>& work, int64_t* qoff);
 
   // XCDs of the device (gfx950: 8, each with its own 4 MB L2).
   constexpr int DEVICE_XCDS = 8;

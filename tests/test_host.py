@@ -131,6 +131,49 @@ def test_cross_tiles_cover_grid_once(nx, ny, R, C):
     assert np.all(seen == 1)
 
 
+@pytest.mark.parametrize("nprob,seed", [(1, 1), (3, 2), (40, 3)])
+def test_xcd_queues_partition_the_tile_list(nprob, seed):
+    """The per-XCD queues of a batched plan are a permutation of its tile list
+    cut into 8 contiguous queues: a supertile unit (G x G tiles of one
+    problem) stays in one queue, its tiles in their given order, and the
+    queues' work is balanced to within the largest unit."""
+    rng = np.random.default_rng(seed)
+    R = 16
+    tiles, work = [], []
+    for b in range(nprob):
+        n = int(rng.integers(20, 700))
+        nt = (n + R - 1) // R
+        for i in range(nt):
+            for j in range(i + 1):
+                tiles.append((i * R, j * R, b))
+                work.append(int(rng.integers(1, 1000)))
+    order_in = np.argsort(-np.array(work), kind="stable")  # longest first, as the planner gives them
+    t = np.array([tiles[k] for k in order_in], dtype=np.int32)
+    w = np.array([work[k] for k in order_in], dtype=np.int64)
+    order = np.zeros(len(t), dtype=np.int64)
+    qoff = np.zeros(9, dtype=np.int64)
+    L.check(L.lib().sbamd_queue_order_host(len(t), t.ctypes.data_as(L.c_i32p), w.ctypes.data_as(L.c_i64p),
+                                           R, R, 8, order.ctypes.data_as(L.c_i64p),
+                                           qoff.ctypes.data_as(L.c_i64p)))
+    assert np.array_equal(np.sort(order), np.arange(len(t)))
+    assert qoff[0] == 0 and qoff[8] == len(t) and np.all(np.diff(qoff) >= 0)
+    queue_of = np.searchsorted(qoff, np.arange(len(t)), side="right") - 1
+    units = {}
+    nunits = {G: len({(p, r // (G * R), c // (G * R)) for r, c, p in t}) for G in (16, 8, 4, 2, 1)}
+    G = next(g for g in (16, 8, 4, 2, 1) if nunits[g] >= 64 or g == 1)
+    for pos, k in enumerate(order):
+        r, c, p = t[k]
+        units.setdefault((p, r // (G * R), c // (G * R)), []).append((queue_of[pos], k))
+    umax = 0
+    for members in units.values():
+        assert len({q for q, _ in members}) == 1
+        ks = [k for _, k in members]
+        assert ks == sorted(ks)  # the unit's tiles keep their order
+        umax = max(umax, int(w[ks].sum()))
+    loads = [int(w[order[qoff[x]:qoff[x + 1]]].sum()) for x in range(8)]
+    assert max(loads) - min(loads) <= umax
+
+
 # ------------------------------------------------------- containers -------
 
 @pytest.mark.parametrize("dtype", [sb.float32, sb.float64])
