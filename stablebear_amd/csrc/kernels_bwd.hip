@@ -182,14 +182,17 @@ namespace sbamd
 
     __device__ __forceinline__ WalkEv walk_load(const BwdEvent* cl, int32_t i)
     {
-      const __attribute__((address_space(4))) BwdEvent* c =
-          (const __attribute__((address_space(4))) BwdEvent*)(cl);
+      // one 32-byte scalar load per event
+      typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
+      const __attribute__((address_space(4))) u32x8* c = (const __attribute__((address_space(4))) u32x8*)(cl);
+      const u32x8 w = c[i];
+      const uint32_t w0 = w.s0, w1 = w.s1, w2 = w.s2, w3 = w.s3, w4 = w.s4, w5 = w.s5, w6 = w.s6, w7 = w.s7;
       WalkEv e;
-      e.t = c[i].t;
-      e.d = c[i].d;
-      e.dt = c[i].dt;
-      e.col = c[i].col;
-      e.pos = c[i].pos;
+      e.t = __builtin_bit_cast(double, (static_cast<uint64_t>(w1) << 32) | w0);
+      e.d = __builtin_bit_cast(double, (static_cast<uint64_t>(w3) << 32) | w2);
+      e.dt = __builtin_bit_cast(double, (static_cast<uint64_t>(w5) << 32) | w4);
+      e.col = static_cast<int32_t>(w6);
+      e.pos = static_cast<int32_t>(w7);
       return e;
     }
 

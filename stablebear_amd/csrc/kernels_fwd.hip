@@ -231,9 +231,9 @@ namespace sbamd
 
         // The wavefront walks until its slowest pair reaches Tb (the largest
         // count), each lane at most its budget: (1) every pair of every lane
-        // up to the smallest of those limits over the wavefront; then (2)
-        // each pair up to the largest count, lanes past their budget masked
-        // off. Four steps per compensated add.
+        // up to the smallest of those limits over the wavefront, eight steps
+        // per compensated add; then (2) each pair up to the largest count,
+        // lanes past their budget masked off, four steps per compensated add.
         uint32_t n_top[NCH], c_min = 0x7fffffffu;
 #pragma unroll
         for (int q = 0; q < NCH; ++q)
@@ -246,13 +246,26 @@ namespace sbamd
         for (int q = 0; q < NCH; ++q)
           n_all = n_all < static_cast<int>(n_top[q]) ? n_all : static_cast<int>(n_top[q]);
         int done = 0;
-        for (; done + 4 <= n_all; done += 4)
+        for (; done + 8 <= n_all; done += 8)
+        {
+          step_all();
+          step_all();
+          step_all();
+          step_all();
+          step_all();
+          step_all();
+          step_all();
+          step_all();
+          flush_all();
+        }
+        if (done + 4 <= n_all)
         {
           step_all();
           step_all();
           step_all();
           step_all();
           flush_all();
+          done += 4;
         }
         if (done + 2 <= n_all)
         {

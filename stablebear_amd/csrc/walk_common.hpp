@@ -185,12 +185,12 @@ namespace sbamd
   // Kahan-compensated FP32 sum over blocks of segments (the forward tile
   // walk). add() chains a block's products into one FP32 partial with FMAs;
   // flush() adds the partial to the compensated sum, which the walk does
-  // every 4 segments in its common steps and every 2 in its tails. A block of
+  // every 8 segments in its common steps and every 4 in its tails. A block of
   // b non-negative terms carries a relative error of at most b u (u = 2^-24)
   // and the compensated sum of the blocks at most 2u more, so an L1 / Lp sum
-  // stays within 6u = 3.6e-7 of the exact sum of its FP32 terms, at half the
-  // per-segment work of a compensated add per segment. fold() moves the sum
-  // into the FP64 accumulator (long pairs, every phase).
+  // stays within 10u = 6.0e-7 of the exact sum of its FP32 terms, at a
+  // fraction of the per-segment work of a compensated add per segment. fold()
+  // moves the sum into the FP64 accumulator (long pairs, every phase).
   struct BlockKahanAcc
   {
     float s = 0.f;
