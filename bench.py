@@ -405,7 +405,7 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32c (f64 compare/dt, f32 products, Kahan-compensated f32 sum over blocks of 4 segments, f64 final sum)",
+        "dtype": "fp32c (f64 compare/dt, f32 products, Kahan-compensated f32 sum over blocks of 8 segments, f64 final sum)",
         "data": "synthetic (seeded: m~U{50..500}, Exp(1) increments, N(0,1) values ending at 0)",
         "config": {"workload": f"L1 pdist, N={n}, m~U{{50..500}}",
                    "pairs": pairs, "segments": plan.segments, "tile": list(plan.shape)},
