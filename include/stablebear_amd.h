@@ -137,7 +137,7 @@ int sbamd_ctx_synchronize(sbamd_ctx* ctx);
 int sbamd_ctx_stream(sbamd_ctx* ctx, void** stream);
 /* Device bytes the host-to-host jobs (sbamd_pairwise_submit) may give their
  * output at once; 0 (the default) = half the free device memory when the job
- * starts. Outputs above it, or above 256 MiB, run in bands of rows whose
+ * starts. Outputs above it, or above 32 MiB, run in bands of rows whose
  * downloads overlap the next bands' compute. Replaces the reference's output
  * sizing from free memory (block_matrix_support.cuh:69-93). */
 int sbamd_ctx_set_output_budget(sbamd_ctx* ctx, uint64_t bytes);
@@ -311,11 +311,12 @@ int sbamd_plan_set_shape(sbamd_plan* plan, const sbamd_tile_shape* shape);
 
 /* ---- asynchronous host-to-host jobs (the reference's StoppableTask) ----
  * Runs set upload, planning, execution and the download into out_host on a
- * worker thread. Outputs within the context's output budget (and 256 MiB)
- * are computed whole on the device and downloaded once; larger ones in bands
- * of consecutive rows (original order), each band's output written in place
- * on the device, downloaded into pinned memory while the next band computes,
- * and copied into out_host. progress counts pairs ("integral" unit).
+ * worker thread. Outputs within the context's output budget (and 32 MiB)
+ * are computed whole on the device and downloaded once; larger ones in at
+ * least 8 bands of consecutive rows (original order), each band's output
+ * written in place on the device, downloaded into pinned memory (kept by the
+ * context for later jobs) while the next bands compute, and copied into
+ * out_host. progress counts pairs ("integral" unit).
  * Replaces create_cuda_block_{integrate_l1,integrate_lp,integrate_l2_kernel,
  * cdist_l1,cdist_lp}_task (cuda_matrix_integrate_api.hpp:19-113) and the
  * StoppableTask wait / progress / cancel protocol (task.hpp:15-193). */

@@ -65,6 +65,52 @@ namespace sbamd
       (void)hipFree(ptr);
   }
 
+  PinnedPool::~PinnedPool()
+  {
+    for (auto& e : idle)
+      (void)hipHostFree(e.first);
+  }
+
+  void* PinnedPool::take(size_t bytes, size_t& cap)
+  {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      size_t best = idle.size();
+      for (size_t k = 0; k < idle.size(); ++k)
+        if (idle[k].second >= bytes && (best == idle.size() || idle[k].second < idle[best].second))
+          best = k;
+      if (best < idle.size())
+      {
+        void* p = idle[best].first;
+        cap = idle[best].second;
+        idle.erase(idle.begin() + static_cast<std::ptrdiff_t>(best));
+        return p;
+      }
+    }
+    void* p = nullptr;
+    SBAMD_HIP(hipHostMalloc(&p, std::max<size_t>(bytes, 1), hipHostMallocDefault));
+    cap = std::max<size_t>(bytes, 1);
+    return p;
+  }
+
+  void PinnedPool::give(void* p, size_t cap)
+  {
+    void* drop = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      idle.emplace_back(p, cap);
+      if (idle.size() > KEEP)
+      {
+        auto it = std::min_element(idle.begin(), idle.end(),
+                                   [](const auto& a, const auto& b) { return a.second < b.second; });
+        drop = it->first;
+        idle.erase(it);
+      }
+    }
+    if (drop)
+      (void)hipHostFree(drop);
+  }
+
   namespace
   {
     thread_local std::string g_last_error;
@@ -172,11 +218,53 @@ namespace sbamd
       return it != live_plans().end() && it->second == gen;
     }
 
+    // Uploads that do not block the host while the device is busy: a
+    // blocking hipMemcpy waits for the kernels already queued on the device
+    // with some HIP runtimes (the one PyTorch bundles does), which would
+    // serialise a job's planning of band k + 1 with band k's kernels. While
+    // an AsyncUploads is installed on a thread, that thread's uploads are
+    // copied into pinned staging leased from the context's pool and sent
+    // with hipMemcpyAsync on the given stream, so they also stay ordered
+    // before the kernels later queued on it. The staging is returned when
+    // the AsyncUploads ends, which must be after that stream has drained.
+    struct AsyncUploads;
+    thread_local AsyncUploads* t_async_uploads = nullptr;
+
+    struct AsyncUploads
+    {
+      static constexpr size_t ARENA = size_t(16) << 20;
+      Ctx* ctx;
+      hipStream_t stream;
+      AsyncUploads* outer;
+      std::vector<std::unique_ptr<PinnedLease>> staging;
+      size_t used = 0;
+      AsyncUploads(Ctx* c, hipStream_t st) : ctx(c), stream(st), outer(t_async_uploads) { t_async_uploads = this; }
+      AsyncUploads(const AsyncUploads&) = delete;
+      AsyncUploads& operator=(const AsyncUploads&) = delete;
+      ~AsyncUploads() { t_async_uploads = outer; }
+      void copy(void* dst, const void* src, size_t bytes)
+      {
+        if (staging.empty() || used + bytes > staging.back()->cap)
+        {
+          staging.push_back(std::make_unique<PinnedLease>(ctx->pinned, std::max(ARENA, bytes)));
+          used = 0;
+        }
+        unsigned char* at = static_cast<unsigned char*>(staging.back()->ptr) + used;
+        std::memcpy(at, src, bytes);
+        SBAMD_HIP(hipMemcpyAsync(dst, at, bytes, hipMemcpyHostToDevice, stream));
+        used += (bytes + 255) & ~size_t(255);
+      }
+    };
+
     template <typename T>
     void upload(DeviceBuffer& buf, const std::vector<T>& v)
     {
       buf = DeviceBuffer(v.size() * sizeof(T));
-      if (!v.empty())
+      if (v.empty())
+        return;
+      if (t_async_uploads)
+        t_async_uploads->copy(buf.ptr, v.data(), v.size() * sizeof(T));
+      else
         SBAMD_HIP(hipMemcpy(buf.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
     }
 
@@ -1289,7 +1377,10 @@ namespace
   // Host-to-host job bodies (the reference's CudaPairwiseIntegrationTask /
   // CudaCrossIntegrationTask, pcf_block_op.cuh:127-286, with the block
   // pipeline of cuda_block_executor.cuh:224-297).
-  constexpr uint64_t JOB_SINGLE_MAX_BYTES = uint64_t(256) << 20;
+  constexpr uint64_t JOB_SINGLE_MAX_BYTES = uint64_t(32) << 20;
+  // Banded jobs: device / pinned buffers in rotation, and the fewest bands
+  constexpr int JOB_BUFFERS = 3;
+  constexpr uint64_t JOB_BANDS = 8;
 
   template <typename T, typename D>
   std::unique_ptr<typename std::remove_pointer<T>::type, D> guard(T p, D d)
@@ -1335,11 +1426,15 @@ namespace
     return sh;
   }
 
-  // Whole output on the device, tile ranges for progress and cancel, one
-  // download at the end (outputs that fit easily).
+  // Whole output on the device (outputs that fit easily), one download at
+  // the end. The tiles run in 64 ranges with an event after each; up to
+  // JOB_AHEAD ranges are queued beyond the one the host waits for, so the
+  // device does not idle while the host counts progress and checks for
+  // cancellation between ranges.
   void job_single(Ctx* ctx, sbamd_job* jp, Set& rs, Set* cs, int32_t op, double p, int32_t out_kind,
                   void* out_host, int32_t out_dtype)
   {
+    constexpr int64_t JOB_AHEAD = 3;
     const sbamd_tile_shape sh = job_shape(ctx);
     auto plan = new_plan(ctx, &rs, cs, op, p, out_kind, &sh, 0);
     const size_t eb = out_dtype == SBAMD_F64 ? 8 : 4;
@@ -1350,40 +1445,66 @@ namespace
     const int64_t nt = work_units(*plan);
     const int64_t chunks = std::min<int64_t>(nt, 64);
     const bool diag = out_kind == SBAMD_OUT_SYMMAT;
-    hipEvent_t ev;
-    SBAMD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    auto eguard = guard(ev, &hipEventDestroy);
-    for (int64_t c = 0; c < chunks; ++c)
+    std::array<hipEvent_t, JOB_AHEAD + 1> ev{};
+    auto ev_free = [&]() {
+      for (auto& e : ev)
+        if (e)
+          (void)hipEventDestroy(e);
+    };
+    auto range = [&](int64_t c) { return std::make_pair(nt * c / chunks, nt * (c + 1) / chunks); };
+    try
     {
-      if (jp->cancel.load())
-        fail(SBAMD_ERR_CANCELLED, "cancelled");
-      const int64_t t0 = nt * c / chunks, t1 = nt * (c + 1) / chunks;
-      execute_range(*plan, dout.ptr, out_dtype, nullptr, t0, t1, st);
-      SBAMD_HIP(hipEventRecord(ev, st));
-      while (hipEventQuery(ev) == hipErrorNotReady)
+      for (auto& e : ev)
+        SBAMD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      int64_t queued = 0;
+      for (int64_t c = 0; c < chunks; ++c)
       {
+        for (; queued < chunks && queued <= c + JOB_AHEAD; ++queued)
+        {
+          const auto r = range(queued);
+          execute_range(*plan, dout.ptr, out_dtype, nullptr, r.first, r.second, st);
+          SBAMD_HIP(hipEventRecord(ev[static_cast<size_t>(queued % (JOB_AHEAD + 1))], st));
+        }
+        hipEvent_t e = ev[static_cast<size_t>(c % (JOB_AHEAD + 1))];
+        while (hipEventQuery(e) == hipErrorNotReady)
+        {
+          if (jp->cancel.load())
+            break;
+          std::this_thread::sleep_for(std::chrono::microseconds(200));
+        }
         if (jp->cancel.load())
-          break;
-        std::this_thread::sleep_for(std::chrono::microseconds(200));
+          fail(SBAMD_ERR_CANCELLED, "cancelled");
+        SBAMD_HIP(hipEventSynchronize(e));
+        const auto r = range(c);
+        uint64_t done = 0;
+        if (plan->use_long)
+          done = static_cast<uint64_t>(r.second - r.first);
+        else
+          for (int64_t k = r.first; k < r.second; ++k)
+            done += tile_pairs(plan->h_tiles[k], diag);
+        jp->progress += done;
       }
       SBAMD_HIP(hipStreamSynchronize(st));
-      uint64_t done = 0;
-      if (plan->use_long)
-        done = static_cast<uint64_t>(t1 - t0);
-      else
-        for (int64_t k = t0; k < t1; ++k)
-          done += tile_pairs(plan->h_tiles[k], diag);
-      jp->progress += done;
+      SBAMD_HIP(hipMemcpy(out_host, dout.ptr, static_cast<size_t>(plan->out_count) * eb, hipMemcpyDeviceToHost));
     }
-    SBAMD_HIP(hipMemcpy(out_host, dout.ptr, static_cast<size_t>(plan->out_count) * eb, hipMemcpyDeviceToHost));
+    catch (...)
+    {
+      (void)hipStreamSynchronize(st);
+      ev_free();
+      throw;
+    }
+    ev_free();
   }
 
   // Row bands in the original order, each writing one contiguous range of
   // the output: rows [r0, r1) against every earlier PCF (a cross plan) and
   // among themselves (a pdist / kernel plan) for the compressed layouts, or
-  // against every column PCF for a dense cdist. Bands alternate between two
-  // device buffers and two pinned host buffers: a band's download (on its
-  // own stream) and its copy into out_host overlap the next bands' compute.
+  // against every column PCF for a dense cdist. Bands rotate over
+  // JOB_BUFFERS device buffers and as many pinned host buffers (kept by the
+  // context between jobs): while the device computes the next bands, a
+  // band's download runs on its own stream and the host copies the band
+  // before it into out_host. Each band's sets and plans are made on the host
+  // while the device works on the bands before it.
   void job_bands(Ctx* ctx, sbamd_job* jp, Set& rs, Set* cs, int32_t op, double p, int32_t out_kind,
                  void* out_host, int32_t out_dtype, uint64_t chunk_elems)
   {
@@ -1397,24 +1518,39 @@ namespace
         return u * static_cast<uint64_t>(ncols);
       return out_kind == SBAMD_OUT_SYMMAT ? u * (u + 1) / 2 : (u > 0 ? u * (u - 1) / 2 : 0);
     };
-    std::vector<int64_t> bounds{0};
-    while (bounds.back() < n)
+    // Bands of near-equal output, at least JOB_BANDS of them and enough that
+    // none exceeds chunk_elems: band k ends at the first row whose prefix
+    // reaches k / nb of the total, so a band holds at most total / nb plus
+    // one row.
+    const uint64_t total = cum(n);
+    const uint64_t widest = cum(n) - cum(n - 1);
+    if (widest > chunk_elems)
+      fail(SBAMD_ERR_NOMEM, "one output row needs more device memory than the output budget allows");
+    const uint64_t per = chunk_elems - widest;
+    const int64_t nb = per == 0 ? n
+                                : static_cast<int64_t>(std::min<uint64_t>(
+                                      std::max<uint64_t>(JOB_BANDS, (total + per - 1) / per), static_cast<uint64_t>(n)));
+    std::vector<int64_t> bounds(static_cast<size_t>(nb) + 1, 0);
+    for (int64_t k = 1; k <= nb; ++k)
     {
-      const int64_t r0 = bounds.back();
-      int64_t lo = r0 + 1, hi = n; // largest r1 with cum(r1) - cum(r0) <= chunk_elems, at least r0 + 1
+      const uint64_t target = k == nb ? total : static_cast<uint64_t>(
+                                                    static_cast<unsigned __int128>(total) * k / nb);
+      int64_t lo = bounds[k - 1], hi = n; // first r with cum(r) >= target
       while (lo < hi)
       {
-        const int64_t mid = lo + (hi - lo + 1) / 2;
-        if (cum(mid) - cum(r0) <= chunk_elems)
-          lo = mid;
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (cum(mid) >= target)
+          hi = mid;
         else
-          hi = mid - 1;
+          lo = mid + 1;
       }
-      if (cum(lo) - cum(r0) > chunk_elems)
-        fail(SBAMD_ERR_NOMEM, "one output row needs more device memory than the output budget allows");
-      bounds.push_back(lo);
+      bounds[k] = lo;
     }
-    const int64_t nb = static_cast<int64_t>(bounds.size()) - 1;
+    bounds.erase(std::unique(bounds.begin(), bounds.end()), bounds.end());
+    const int64_t nbands = static_cast<int64_t>(bounds.size()) - 1;
+    for (int64_t k = 0; k < nbands; ++k)
+      if (cum(bounds[k + 1]) - cum(bounds[k]) > chunk_elems)
+        fail(SBAMD_ERR_RUNTIME, "band split exceeds the output budget");
     const sbamd_tile_shape sh = job_shape(ctx);
 
     hipStream_t st, cst;
@@ -1422,37 +1558,35 @@ namespace
     auto g1 = guard(st, &hipStreamDestroy);
     SBAMD_HIP(hipStreamCreateWithFlags(&cst, hipStreamNonBlocking));
     auto g2 = guard(cst, &hipStreamDestroy);
-    const size_t buf_bytes = static_cast<size_t>(chunk_elems) * eb;
-    DeviceBuffer dbuf[2] = {DeviceBuffer(buf_bytes), DeviceBuffer(buf_bytes)};
-    void* hbuf[2] = {nullptr, nullptr};
-    SBAMD_HIP(hipHostMalloc(&hbuf[0], buf_bytes, hipHostMallocDefault));
-    auto h0 = guard(hbuf[0], &hipHostFree);
-    SBAMD_HIP(hipHostMalloc(&hbuf[1], buf_bytes, hipHostMallocDefault));
-    auto h1 = guard(hbuf[1], &hipHostFree);
-    hipEvent_t done_ev[2], copied_ev[2];
-    for (int b = 0; b < 2; ++b)
+    uint64_t widest_band = 0;
+    for (int64_t k = 0; k < nbands; ++k)
+      widest_band = std::max<uint64_t>(widest_band, cum(bounds[k + 1]) - cum(bounds[k]));
+    const size_t buf_bytes = static_cast<size_t>(widest_band) * eb;
+    const int nbuf = static_cast<int>(std::min<int64_t>(JOB_BUFFERS, nbands));
+    std::vector<DeviceBuffer> dbuf;
+    std::vector<std::unique_ptr<PinnedLease>> hbuf;
+    for (int b = 0; b < nbuf; ++b)
     {
-      SBAMD_HIP(hipEventCreateWithFlags(&done_ev[b], hipEventDisableTiming));
-      SBAMD_HIP(hipEventCreateWithFlags(&copied_ev[b], hipEventDisableTiming));
+      dbuf.emplace_back(buf_bytes);
+      hbuf.push_back(std::make_unique<PinnedLease>(ctx->pinned, buf_bytes));
     }
+    std::vector<hipEvent_t> done_ev(static_cast<size_t>(nbuf), nullptr), copied_ev(static_cast<size_t>(nbuf), nullptr);
     auto ev_free = [&]() {
-      for (int b = 0; b < 2; ++b)
+      for (int b = 0; b < nbuf; ++b)
       {
-        (void)hipEventDestroy(done_ev[b]);
-        (void)hipEventDestroy(copied_ev[b]);
+        if (done_ev[b])
+          (void)hipEventDestroy(done_ev[b]);
+        if (copied_ev[b])
+          (void)hipEventDestroy(copied_ev[b]);
       }
     };
-    // Every band's sets and plans are made before the first launch when
-    // their records fit in a quarter of the free memory (freeing device
-    // memory between launches would wait for the device); otherwise each
-    // band is made just before its launch.
     struct Band
     {
       Set rows, prefix;
       DeviceBuffer sel_rows, sel_prefix;
       std::unique_ptr<sbamd_plan> cross, self;
     };
-    std::vector<std::unique_ptr<Band>> band(static_cast<size_t>(nb));
+    std::vector<std::unique_ptr<Band>> band(static_cast<size_t>(nbands));
     const int rb = record_bytes(rs.tt_f64);
     uint64_t all_bytes = 0;
     {
@@ -1462,7 +1596,7 @@ namespace
         mrec[static_cast<size_t>(rs.h_perm[q])] = rs.h_roff[q + 1] - rs.h_roff[q];
       uint64_t below = 0;
       int64_t r = 0;
-      for (int64_t k = 0; k < nb; ++k)
+      for (int64_t k = 0; k < nbands; ++k)
       {
         const uint64_t pre = below;
         for (; r < bounds[k + 1]; ++r)
@@ -1470,9 +1604,12 @@ namespace
         all_bytes += (below - pre + (cs ? 0 : pre)) * static_cast<uint64_t>(rb);
       }
     }
+    // Bands whose records all fit in a quarter of the free memory are kept
+    // until the job ends (freeing device memory between launches waits for
+    // the device); otherwise each is freed once its output is copied out.
     size_t free_b = 0, total_b = 0;
     SBAMD_HIP(hipMemGetInfo(&free_b, &total_b));
-    const bool prebuild = all_bytes <= static_cast<uint64_t>(free_b) / 4;
+    const bool keep = all_bytes <= static_cast<uint64_t>(free_b) / 4;
     auto build = [&](int64_t k) {
       const int64_t r0 = bounds[k], r1 = bounds[k + 1];
       const int64_t base = -static_cast<int64_t>(cum(r0));
@@ -1502,44 +1639,56 @@ namespace
       }
       band[static_cast<size_t>(k)] = std::move(bd);
     };
-    // band k - 2 used buffer k & 1: wait for its download and copy it out
+    // band k used buffer k % nbuf: wait for its download and copy it out
     auto retire = [&](int64_t k) {
       if (k < 0)
         return;
-      const int b = static_cast<int>(k & 1);
+      const int b = static_cast<int>(k % nbuf);
       SBAMD_HIP(hipEventSynchronize(copied_ev[b]));
       const uint64_t e0 = cum(bounds[k]), e1 = cum(bounds[k + 1]);
-      parallel_copy(static_cast<unsigned char*>(out_host) + e0 * eb, hbuf[b], static_cast<size_t>(e1 - e0) * eb);
+      parallel_copy(static_cast<unsigned char*>(out_host) + e0 * eb, hbuf[b]->ptr, static_cast<size_t>(e1 - e0) * eb);
       jp->progress += e1 - e0;
-      if (!prebuild)
+      if (!keep)
         band[static_cast<size_t>(k)].reset();
     };
+    // Band k's download, issued once its kernels have finished (the host
+    // waits while the device runs band k + 1): with a cross-stream event
+    // wait instead, hipMemcpyAsync blocked the host on some HIP runtimes.
+    auto download = [&](int64_t k) {
+      const int b = static_cast<int>(k % nbuf);
+      SBAMD_HIP(hipEventSynchronize(done_ev[b]));
+      SBAMD_HIP(hipMemcpyAsync(hbuf[b]->ptr, dbuf[b].ptr, static_cast<size_t>(cum(bounds[k + 1]) - cum(bounds[k])) * eb,
+                               hipMemcpyDeviceToHost, cst));
+      SBAMD_HIP(hipEventRecord(copied_ev[b], cst));
+    };
+    // band plans are made while the device runs the bands before them
+    AsyncUploads uploads(ctx, st);
     try
     {
-      if (prebuild)
-        for (int64_t k = 0; k < nb; ++k)
-          build(k);
-      for (int64_t k = 0; k < nb; ++k)
+      for (int b = 0; b < nbuf; ++b)
       {
-        const int b = static_cast<int>(k & 1);
-        retire(k - 2);
+        SBAMD_HIP(hipEventCreateWithFlags(&done_ev[b], hipEventDisableTiming));
+        SBAMD_HIP(hipEventCreateWithFlags(&copied_ev[b], hipEventDisableTiming));
+      }
+      for (int64_t k = 0; k < nbands; ++k)
+      {
+        const int b = static_cast<int>(k % nbuf);
         if (jp->cancel.load())
           fail(SBAMD_ERR_CANCELLED, "cancelled");
-        if (!prebuild)
-          build(k);
+        build(k);
+        retire(k - nbuf);
         Band& bd = *band[static_cast<size_t>(k)];
         if (bd.cross)
           execute_range(*bd.cross, dbuf[b].ptr, out_dtype, nullptr, 0, work_units(*bd.cross), st);
         if (bd.self)
           execute_range(*bd.self, dbuf[b].ptr, out_dtype, nullptr, 0, work_units(*bd.self), st);
         SBAMD_HIP(hipEventRecord(done_ev[b], st));
-        SBAMD_HIP(hipStreamWaitEvent(cst, done_ev[b], 0));
-        SBAMD_HIP(hipMemcpyAsync(hbuf[b], dbuf[b].ptr, static_cast<size_t>(cum(bounds[k + 1]) - cum(bounds[k])) * eb,
-                                 hipMemcpyDeviceToHost, cst));
-        SBAMD_HIP(hipEventRecord(copied_ev[b], cst));
+        if (k > 0)
+          download(k - 1);
       }
-      retire(nb - 2);
-      retire(nb - 1);
+      download(nbands - 1);
+      for (int64_t k = std::max<int64_t>(0, nbands - nbuf); k < nbands; ++k)
+        retire(k);
       SBAMD_HIP(hipStreamSynchronize(st));
     }
     catch (...)
@@ -1576,9 +1725,8 @@ namespace
       job_single(ctx, jp, rs, cols ? &cs : nullptr, op, p, out_kind, out_host, out_dtype);
       return;
     }
-    // two band buffers within the budget, and at least eight bands for overlap
-    uint64_t chunk = std::min<uint64_t>(budget / (2 * eb), (out_count + 7) / 8);
-    chunk = std::max<uint64_t>(chunk, 1);
+    // JOB_BUFFERS band buffers within the budget
+    const uint64_t chunk = std::max<uint64_t>(budget / (JOB_BUFFERS * eb), 1);
     job_bands(ctx, jp, rs, cols ? &cs : nullptr, op, p, out_kind, out_host, out_dtype, chunk);
   }
 }

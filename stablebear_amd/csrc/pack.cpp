@@ -9,8 +9,9 @@
 #include <cmath>
 #include <limits>
 #include <numeric>
-#include <sstream>
 #include <memory>
+#include <sstream>
+#include <thread>
 #include <unordered_map>
 
 namespace sbamd
@@ -61,14 +62,73 @@ namespace sbamd
         fail(SBAMD_ERR_INVALID, "PCF set offsets must be non-decreasing");
   }
 
+  namespace
+  {
+    // First PCF of [b, e) that breaks a rule of validate_set, or -1.
+    template <typename T>
+    int64_t first_bad(const sbamd_pcf_set& s, int64_t b, int64_t e)
+    {
+      const T* t = static_cast<const T*>(s.t);
+      for (int64_t i = b; i < e; ++i)
+      {
+        const int64_t k0 = s.offsets[i], k1 = s.offsets[i + 1];
+        if (k0 == k1)
+          continue;
+        if (t[k0] != T(0))
+          return i;
+        T prev = t[k0];
+        for (int64_t k = k0 + 1; k < k1; ++k)
+        {
+          const T tk = t[k];
+          // rejects NaN and +-inf as well: only finite, non-decreasing times pass
+          if (!(tk >= prev) || !(tk <= std::numeric_limits<T>::max()))
+            return i;
+          prev = tk;
+        }
+      }
+      return -1;
+    }
+  }
+
   // Breakpoint rules of the reference's Pcf constructor
   // (src/python/pypcf_support.hpp:44-87): an empty PCF is the zero PCF;
   // times non-decreasing; first time exactly 0. Times must also be finite
-  // (the walk's end-of-PCF sentinel is +inf).
+  // (the walk's end-of-PCF sentinel is +inf). Large sets are scanned by up
+  // to 16 threads over ranges of about equal points; the message is that of
+  // the first offending PCF, as in a serial scan.
   void validate_set(const sbamd_pcf_set& s)
   {
     check_header(s);
-    for (int64_t i = 0; i < s.n; ++i)
+    const int64_t M = s.n > 0 ? s.offsets[s.n] : 0;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nth = M >= (int64_t(1) << 20) ? static_cast<int>(std::min(16u, hw)) : 1;
+    int64_t from = 0, to = s.n;
+    if (nth > 1)
+    {
+      std::vector<int64_t> cut(static_cast<size_t>(nth) + 1, s.n), bad(static_cast<size_t>(nth), -1);
+      cut[0] = 0;
+      for (int k = 1; k < nth; ++k)
+        cut[k] = std::lower_bound(s.offsets, s.offsets + s.n, M * k / nth) - s.offsets;
+      auto scan = [&](int k) {
+        bad[k] = s.dtype == SBAMD_F64 ? first_bad<double>(s, cut[k], cut[k + 1])
+                                      : first_bad<float>(s, cut[k], cut[k + 1]);
+      };
+      std::vector<std::thread> th;
+      for (int k = 1; k < nth; ++k)
+        th.emplace_back(scan, k);
+      scan(0);
+      for (auto& x : th)
+        x.join();
+      from = to = s.n;
+      for (int k = 0; k < nth; ++k)
+        if (bad[k] >= 0)
+        {
+          from = bad[k];
+          to = bad[k] + 1;
+          break;
+        }
+    }
+    for (int64_t i = from; i < to; ++i)
     {
       const int64_t b = s.offsets[i], e = s.offsets[i + 1];
       if (e == b)

@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/stablebear_amd.h"
@@ -234,6 +236,40 @@ namespace sbamd
   std::vector<int64_t> record_counts(const sbamd_pcf_set& in);
   void sort_by_records(const std::vector<int64_t>& m, std::vector<int64_t>& perm);
 
+  // Pinned host buffers that host-to-host jobs stage their transfers
+  // through, kept for the next job: pinning (and first touching) hundreds of
+  // MB costs more than the copies they serve. At most KEEP idle buffers are
+  // held; the smallest goes first.
+  struct PinnedPool
+  {
+    static constexpr size_t KEEP = 4;
+    std::mutex mu;
+    std::vector<std::pair<void*, size_t>> idle;
+    PinnedPool() = default;
+    PinnedPool(const PinnedPool&) = delete;
+    PinnedPool& operator=(const PinnedPool&) = delete;
+    ~PinnedPool();
+    void* take(size_t bytes, size_t& cap); // an idle buffer of at least `bytes`, else a new one
+    void give(void* p, size_t cap);
+  };
+
+  // One pinned buffer of a pool for the lifetime of the lease.
+  struct PinnedLease
+  {
+    PinnedPool* pool = nullptr;
+    void* ptr = nullptr;
+    size_t cap = 0;
+    PinnedLease() = default;
+    PinnedLease(PinnedPool& pl, size_t bytes) : pool(&pl) { ptr = pl.take(bytes, cap); }
+    PinnedLease(const PinnedLease&) = delete;
+    PinnedLease& operator=(const PinnedLease&) = delete;
+    ~PinnedLease()
+    {
+      if (ptr)
+        pool->give(ptr, cap);
+    }
+  };
+
   struct Ctx
   {
     int device = 0;
@@ -246,6 +282,7 @@ namespace sbamd
     // free memory) and the tile shape of their plans (0 fields: defaults)
     uint64_t out_budget = 0;
     int32_t job_R = 0, job_C = 0, job_cap = 0, job_ppl = 0;
+    PinnedPool pinned;
   };
 
   struct Set

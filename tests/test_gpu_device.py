@@ -283,7 +283,7 @@ def test_job_bands_under_small_output_budget(gpu, kind):
         op, okind, out_kind = L.OP_L1, oracle.KIND_PDIST, L.OUT_DISTMAT
         exp = oracle.pairwise(oracle.OP_L1, 1.0, okind, off, t, v)
     try:
-        runtime.set_output_budget(total * 8 // 9)  # bands of ~1/18 of the output, two buffers
+        runtime.set_output_budget(total * 8 // 9)  # three buffers: bands of ~1/27 of the output
         out = np.full(total, np.nan)
         task = runtime.submit(csr, cols, op, 1.0, out_kind, L.PREC_FP64, out, "d", total)
         task.wait()

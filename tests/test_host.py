@@ -66,6 +66,28 @@ def test_validation_rules():
         _validate(bad)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_validation_of_large_sets_reports_the_first_offending_pcf(dtype):
+    """Sets of 2^20 points or more are scanned by several threads; the
+    error is still the one of the lowest offending PCF, for each rule."""
+    n, m = 3000, 500  # 1.5e6 points
+    off = np.arange(n + 1, dtype=np.int64) * m
+    t = np.tile(np.arange(m, dtype=dtype), n)
+    v = np.ones(n * m, dtype=dtype)
+    _validate(runtime.csr_from_arrays(off, t, v))
+    cases = [("non-decreasing", lambda t, i: t.__setitem__(off[i] + 7, t[off[i] + 5])),
+             ("t=0", lambda t, i: t.__setitem__(off[i], 0.5)),
+             ("finite", lambda t, i: t.__setitem__(off[i] + 9, np.inf)),
+             ("finite", lambda t, i: t.__setitem__(off[i] + 9, np.nan))]
+    for (first_msg, first), (_, later) in zip(cases, cases[1:] + cases[:1]):
+        tt = t.copy()
+        later(tt, 2900)  # in the last thread's range
+        first(tt, 1700)
+        later(tt, 2100)
+        with pytest.raises(ValueError, match=first_msg):
+            _validate(runtime.csr_from_arrays(off, tt, v))
+
+
 def test_count_segments_formula_and_brute_force():
     rng = np.random.default_rng(4)
     pcfs = []
