@@ -100,7 +100,9 @@ typedef struct
  * (rows + cols <= 64), `cap` breakpoint records per PCF staged in LDS per
  * phase, `lane_pairs` pairs walked by each lane (1 or 2; a workgroup has
  * rows * ceil(cols / lane_pairs) lanes, at most 512). Zero fields mean "use
- * the tuned or default value". */
+ * the tuned or default value"; for a pdist / kernel plan, rows without cols
+ * means a square tile. Tiles of pdist / kernel plans may be rectangular:
+ * those that reach the diagonal skip the pairs on or above it. */
 typedef struct
 {
   int32_t rows;
@@ -421,9 +423,12 @@ int sbamd_validate_set(const sbamd_pcf_set* set);
 int sbamd_count_segments(const sbamd_pcf_set* rows, const sbamd_pcf_set* cols,
                          int32_t include_diagonal, uint64_t* segments);
 /* Tile list the planner would build for record counts m_rows (n_rows) and
- * m_cols (n_cols, or NULL for the symmetric lower triangle): writes up to cap
+ * m_cols (n_cols, or NULL for the strict lower triangle): writes up to cap
  * tiles as (row0, col0, nrows, ncols) int32 quadruples in sorted index space
- * plus the sorted permutation of rows (perm_rows, n_rows entries). */
+ * plus the sorted permutation of rows (perm_rows, n_rows entries). In the
+ * triangle, ncols carries 1 << 16 when the tile also holds pairs on or
+ * above the diagonal (the kernel then skips those); tiles may be
+ * rectangular there too. */
 int sbamd_plan_tiles_host(int64_t n_rows, const int64_t* m_rows, int64_t n_cols,
                           const int64_t* m_cols, int32_t tile_rows, int32_t tile_cols,
                           int32_t* tiles, int64_t cap, int64_t* n_tiles, int64_t* perm_rows);

@@ -813,8 +813,14 @@ namespace sbamd
     {
       if (t.flags & TILE_DIAG)
       {
-        const uint64_t n = static_cast<uint64_t>(t.nr);
-        return include_diag ? n * (n + 1) / 2 : n * (n - 1) / 2;
+        // pairs (r0 + a, c0 + b) with c0 + b <= r0 + a (- 1 without the diagonal)
+        uint64_t n = 0;
+        for (int a = 0; a < t.nr; ++a)
+        {
+          const int64_t upto = static_cast<int64_t>(t.r0) + a - (include_diag ? 0 : 1) - t.c0 + 1;
+          n += static_cast<uint64_t>(std::min<int64_t>(std::max<int64_t>(upto, 0), t.nc));
+        }
+        return n;
       }
       return static_cast<uint64_t>(t.nr) * static_cast<uint64_t>(t.nc);
     }
@@ -934,6 +940,24 @@ namespace sbamd
     // dense matrix; the host-to-host jobs also make cross plans that write
     // compressed outputs (pairs whose row index exceeds every column index).
     // out_base is added to every output index.
+    // Default tile: 48 slots of 64 staged records (three workgroups of 8
+    // wavefronts per CU, set by LDS): 16 rows x 32 columns for a pdist /
+    // kernel plan, 32 x 16 for a cdist (measured best on MI355X for m ~
+    // 50..500: fewer phases per tile than 16 x 16 x 48 at the same
+    // occupancy); 16 x 16 x 48 where a workgroup cannot hold that much LDS.
+    void set_default_shape(Plan& pl)
+    {
+      pl.R = pl.cdist ? 32 : 16;
+      pl.C = pl.cdist ? 16 : 32;
+      pl.cap = 64;
+      pl.ppl = 1;
+      if (fwd_lds_bytes(pl.tt_f64, pl.tv_f64, pl.R, pl.C, pl.cap) > pl.ctx->lds_per_block)
+      {
+        pl.R = pl.C = 16;
+        pl.cap = 48;
+      }
+    }
+
     std::unique_ptr<sbamd_plan> new_plan(Ctx* ctx, const Set* rows, const Set* cols, int32_t op, double p,
                                          int32_t out_kind, const sbamd_tile_shape* shape, int64_t out_base)
     {
@@ -984,19 +1008,20 @@ namespace sbamd
       pl->input_bytes = 0;
       for (const Set* st : pl->sets)
         pl->input_bytes += st->rec.bytes + st->roff.bytes;
+      set_default_shape(*pl);
       if (shape)
       {
         if (shape->rows)
           pl->R = shape->rows;
         if (shape->cols)
           pl->C = shape->cols;
+        else if (shape->rows && !pl->cdist)
+          pl->C = pl->R; // only a side given for a symmetric plan: a square tile
         if (shape->cap)
           pl->cap = shape->cap;
         if (shape->lane_pairs)
           pl->ppl = shape->lane_pairs;
       }
-      if (!pl->cdist)
-        pl->C = pl->R;
       check_shape(pl->R, pl->C, pl->cap, pl->ppl);
       pl->rebuild_tiles();
       if (out_base != 0)
@@ -1031,8 +1056,6 @@ namespace sbamd
       for (int R : sides)
         for (int C : sides)
         {
-          if (!plan.cdist && R != C)
-            continue;
           if ((R < 16 || C < 16) && !small)
             continue;
           for (int ppl = 1; ppl <= 2; ++ppl)
@@ -1323,9 +1346,9 @@ namespace sbamd
           {
             const auto& a = at[b];
             const uint16_t prob = static_cast<uint16_t>(b);
-            build_tiles_region(ms[b], ms[b], a[k], a[k + 1], 0, a[k], sh.R, sh.R, false, diag, prob,
+            build_tiles_region(ms[b], ms[b], a[k], a[k + 1], 0, a[k], sh.R, sh.C, false, diag, prob,
                                fold_threshold, h_tiles);
-            build_tiles_region(ms[b], ms[b], a[k], a[k + 1], a[k], a[k + 1], sh.R, sh.R, true, diag, prob,
+            build_tiles_region(ms[b], ms[b], a[k], a[k + 1], a[k], a[k + 1], sh.R, sh.C, true, diag, prob,
                                fold_threshold, h_tiles);
           }
         const std::vector<int64_t> work = lpt(b0, h_tiles.size());
@@ -1633,9 +1656,7 @@ namespace
           make_subset(rs, pre, bd->prefix, bd->sel_prefix, st);
           bd->cross = new_plan(ctx, &bd->rows, &bd->prefix, op, p, out_kind, &sh, base);
         }
-        sbamd_tile_shape sq = sh;
-        sq.cols = 0;
-        bd->self = new_plan(ctx, &bd->rows, nullptr, op, p, out_kind, &sq, base);
+        bd->self = new_plan(ctx, &bd->rows, nullptr, op, p, out_kind, &sh, base);
       }
       band[static_cast<size_t>(k)] = std::move(bd);
     };
@@ -2137,16 +2158,20 @@ extern "C"
       upload(pl->b_roff, roff_all);
       upload(pl->b_perm, perm_all);
       pl->input_bytes = pl->b_rec.bytes + pl->b_roff.bytes;
+      set_default_shape(*pl);
       if (shape)
       {
         if (shape->rows)
           pl->R = shape->rows;
+        if (shape->cols)
+          pl->C = shape->cols;
+        else if (shape->rows)
+          pl->C = pl->R; // only a side given: a square tile
         if (shape->cap)
           pl->cap = shape->cap;
         if (shape->lane_pairs)
           pl->ppl = shape->lane_pairs;
       }
-      pl->C = pl->R;
       check_shape(pl->R, pl->C, pl->cap, pl->ppl);
       pl->rebuild_tiles();
       *out = pl.release();
@@ -2239,11 +2264,9 @@ extern "C"
       need(plan, "plan");
       need(shape, "shape");
       int R = shape->rows ? shape->rows : plan->R;
-      int C = shape->cols ? shape->cols : plan->C;
+      int C = shape->cols ? shape->cols : (shape->rows && !plan->cdist ? R : plan->C);
       int cap = shape->cap ? shape->cap : plan->cap;
       int ppl = shape->lane_pairs ? shape->lane_pairs : plan->ppl;
-      if (!plan->cdist)
-        C = R;
       check_shape(R, C, cap, ppl);
       plan->R = R;
       plan->C = C;
@@ -2679,11 +2702,10 @@ extern "C"
       Shape sh[NCLASS];
       for (int k = 0; k < NCLASS; ++k)
       {
-        sh[k] = Shape{shapes[k].rows ? shapes[k].rows : plan->R, shapes[k].cols ? shapes[k].cols : plan->C,
+        const int R = shapes[k].rows ? shapes[k].rows : plan->R;
+        sh[k] = Shape{R, shapes[k].cols ? shapes[k].cols : (shapes[k].rows && !plan->cdist ? R : plan->C),
                       shapes[k].cap ? shapes[k].cap : plan->cap,
                       shapes[k].lane_pairs ? shapes[k].lane_pairs : plan->ppl};
-        if (!plan->cdist)
-          sh[k].C = sh[k].R;
         check_shape(sh[k].R, sh[k].C, sh[k].cap, sh[k].ppl);
       }
       for (int k = 0; k < NCLASS; ++k)
@@ -3046,8 +3068,8 @@ extern "C"
       }
       else
       {
-        check_shape(tile_rows, tile_rows, 64, 2);
-        build_tiles(mrs, nullptr, tile_rows, tile_rows, false, 0, std::numeric_limits<int64_t>::max(), out);
+        check_shape(tile_rows, tile_cols, 64, 2);
+        build_tiles(mrs, nullptr, tile_rows, tile_cols, false, 0, std::numeric_limits<int64_t>::max(), out);
       }
       *n_tiles = static_cast<int64_t>(out.size());
       if (tiles)
@@ -3056,7 +3078,7 @@ extern "C"
           tiles[4 * k + 0] = out[k].r0;
           tiles[4 * k + 1] = out[k].c0;
           tiles[4 * k + 2] = out[k].nr;
-          tiles[4 * k + 3] = out[k].nc;
+          tiles[4 * k + 3] = out[k].nc | ((out[k].flags & TILE_DIAG) ? (1 << 16) : 0);
         }
       if (perm_rows)
         for (size_t q = 0; q < pr.size(); ++q)

@@ -109,8 +109,10 @@ namespace sbamd
       {
         const int c = lc + q * half;
         bool v = lr < tl.nr && c < tl.nc;
+        // tiles that reach the diagonal of a symmetric plan: only pairs
+        // below it (or on it, for kernel matrices), in sorted positions
         if (v && (tl.flags & TILE_DIAG))
-          v = P.include_diag ? (lr >= c) : (lr > c);
+          v = P.include_diag ? (tl.r0 + lr >= tl.c0 + c) : (tl.r0 + lr > tl.c0 + c);
         valid[q] = v;
         slot_f[q] = v ? lr : 0;
         slot_g[q] = v ? P.R + c : P.R;

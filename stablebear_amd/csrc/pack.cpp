@@ -231,101 +231,89 @@ namespace sbamd
   // space, longest tiles first: rows and columns are sorted by record count,
   // so descending block indices put the longest walks at the head of the
   // launch and the short ones fill the tail.
-  void build_tiles(const std::vector<int64_t>& m_rows, const std::vector<int64_t>* m_cols, int R,
-                   int C, bool include_diag, uint16_t prob, int64_t fold_threshold,
-                   std::vector<Tile>& tiles)
+  namespace
   {
-    const int64_t nr = static_cast<int64_t>(m_rows.size());
-    const bool sym = m_cols == nullptr;
-    const std::vector<int64_t>& mc = sym ? m_rows : *m_cols;
-    const int64_t nc = static_cast<int64_t>(mc.size());
-    if (sym && R != C)
-      fail(SBAMD_ERR_INVALID, "symmetric plans need square tiles");
-    const int64_t nbr = (nr + R - 1) / R;
-    const int64_t nbc = (nc + C - 1) / C;
-    // longest PCF of every row and column block, once
-    auto block_maxima = [](const std::vector<int64_t>& m, int64_t n, int B, int64_t nb) {
-      std::vector<int64_t> mx(static_cast<size_t>(nb), 0);
-      for (int64_t k = 0; k < n; ++k)
-        mx[static_cast<size_t>(k / B)] = std::max(mx[static_cast<size_t>(k / B)], m[static_cast<size_t>(k)]);
-      return mx;
-    };
-    const std::vector<int64_t> rmx = block_maxima(m_rows, nr, R, nbr);
-    const std::vector<int64_t> cmx = sym ? rmx : block_maxima(mc, nc, C, nbc);
-    tiles.reserve(tiles.size() + static_cast<size_t>(sym ? nbr * (nbr + 1) / 2 : nbr * nbc));
-    for (int64_t bi = nbr - 1; bi >= 0; --bi)
+    // Tiles of R x C pairs over rows [r_lo, r_hi) x cols [c_lo, c_hi) of
+    // sorted index space, longest first (the rows and columns are sorted by
+    // record count, so the last blocks hold the longest PCFs). With `tri`
+    // (r_lo == c_lo: a block of a symmetric plan) only pairs (a, b) with
+    // a > b, or a >= b with include_diag, are wanted: a tile is kept when it
+    // holds one, and flagged TILE_DIAG when it also holds an unwanted pair
+    // (the kernel then checks each pair). R and C need not be equal.
+    void tile_grid(const std::vector<int64_t>& m_rows, const std::vector<int64_t>& m_cols, int64_t r_lo,
+                   int64_t r_hi, int64_t c_lo, int64_t c_hi, int R, int C, bool tri, bool include_diag,
+                   uint16_t prob, int64_t fold_threshold, std::vector<Tile>& tiles)
     {
-      const int64_t r0 = bi * R;
-      const int64_t rn = std::min<int64_t>(R, nr - r0);
-      const int64_t rmax = rmx[static_cast<size_t>(bi)];
-      const int64_t bj_end = sym ? bi : nbc - 1;
-      for (int64_t bj = bj_end; bj >= 0; --bj)
+      if (r_hi <= r_lo || c_hi <= c_lo)
+        return;
+      const int64_t nbr = (r_hi - r_lo + R - 1) / R;
+      const int64_t nbc = (c_hi - c_lo + C - 1) / C;
+      // longest PCF of every row and column block, once
+      auto block_maxima = [](const std::vector<int64_t>& m, int64_t lo, int64_t hi, int B, int64_t nb) {
+        std::vector<int64_t> mx(static_cast<size_t>(nb), 0);
+        for (int64_t k = lo; k < hi; ++k)
+        {
+          const size_t blk = static_cast<size_t>((k - lo) / B);
+          mx[blk] = std::max(mx[blk], m[static_cast<size_t>(k)]);
+        }
+        return mx;
+      };
+      const std::vector<int64_t> rmx = block_maxima(m_rows, r_lo, r_hi, R, nbr);
+      const std::vector<int64_t> cmx = block_maxima(m_cols, c_lo, c_hi, C, nbc);
+      const int64_t below = include_diag ? 0 : 1; // a wanted pair has col <= row - below
+      for (int64_t bi = nbr - 1; bi >= 0; --bi)
       {
-        const int64_t c0 = bj * C;
-        const int64_t cn = std::min<int64_t>(C, nc - c0);
-        const bool diag = sym && bi == bj;
-        if (diag && !include_diag && rn < 2)
-          continue;
-        Tile t;
-        t.r0 = static_cast<int32_t>(r0);
-        t.c0 = static_cast<int32_t>(c0);
-        t.nr = static_cast<int16_t>(rn);
-        t.nc = static_cast<int16_t>(cn);
-        t.prob = prob;
-        t.flags = diag ? TILE_DIAG : 0;
-        if (rmax + cmx[static_cast<size_t>(bj)] - 1 >= fold_threshold)
-          t.flags |= TILE_FOLD;
-        tiles.push_back(t);
+        const int64_t r0 = r_lo + bi * R;
+        const int64_t rn = std::min<int64_t>(R, r_hi - r0);
+        int64_t bj_end = nbc - 1;
+        if (tri)
+        {
+          // the last column block holding a wanted pair of this row block
+          const int64_t cmax = r0 + rn - 1 - below;
+          if (cmax < c_lo)
+            continue;
+          bj_end = std::min<int64_t>(bj_end, (cmax - c_lo) / C);
+        }
+        for (int64_t bj = bj_end; bj >= 0; --bj)
+        {
+          const int64_t c0 = c_lo + bj * C;
+          const int64_t cn = std::min<int64_t>(C, c_hi - c0);
+          Tile t;
+          t.r0 = static_cast<int32_t>(r0);
+          t.c0 = static_cast<int32_t>(c0);
+          t.nr = static_cast<int16_t>(rn);
+          t.nc = static_cast<int16_t>(cn);
+          t.prob = prob;
+          // an unwanted pair: the first row against the last column
+          t.flags = tri && c0 + cn - 1 > r0 - below ? TILE_DIAG : 0;
+          if (rmx[static_cast<size_t>(bi)] + cmx[static_cast<size_t>(bj)] - 1 >= fold_threshold)
+            t.flags |= TILE_FOLD;
+          tiles.push_back(t);
+        }
       }
     }
   }
 
-  // Tiles of R x C pairs over rows [r_lo, r_hi) x cols [c_lo, c_hi) of sorted
-  // index space (absolute indices). With `tri` the region is the square
-  // r_lo == c_lo, r_hi == c_hi of a symmetric plan and only its lower
-  // triangle is tiled (square tiles, diagonal tiles flagged). Longest tiles
-  // first, as build_tiles.
+  void build_tiles(const std::vector<int64_t>& m_rows, const std::vector<int64_t>* m_cols, int R,
+                   int C, bool include_diag, uint16_t prob, int64_t fold_threshold,
+                   std::vector<Tile>& tiles)
+  {
+    const bool sym = m_cols == nullptr;
+    const std::vector<int64_t>& mc = sym ? m_rows : *m_cols;
+    tile_grid(m_rows, mc, 0, static_cast<int64_t>(m_rows.size()), 0, static_cast<int64_t>(mc.size()), R, C, sym,
+              include_diag, prob, fold_threshold, tiles);
+  }
+
+  // Tiles over rows [r_lo, r_hi) x cols [c_lo, c_hi) of sorted index space
+  // (absolute indices). With `tri` the region is the square r_lo == c_lo,
+  // r_hi == c_hi of a symmetric plan and only its lower triangle is tiled.
   void build_tiles_region(const std::vector<int64_t>& m_rows, const std::vector<int64_t>& m_cols,
                           int64_t r_lo, int64_t r_hi, int64_t c_lo, int64_t c_hi, int R, int C, bool tri,
                           bool include_diag, uint16_t prob, int64_t fold_threshold, std::vector<Tile>& tiles)
   {
-    if (r_hi <= r_lo || c_hi <= c_lo)
-      return;
-    if (tri && R != C)
-      fail(SBAMD_ERR_INVALID, "symmetric plans need square tiles");
-    const int64_t nbr = (r_hi - r_lo + R - 1) / R;
-    const int64_t nbc = (c_hi - c_lo + C - 1) / C;
-    auto block_max = [](const std::vector<int64_t>& m, int64_t b0, int64_t cnt) {
-      int64_t mx = 0;
-      for (int64_t k = b0; k < b0 + cnt; ++k)
-        mx = std::max(mx, m[k]);
-      return mx;
-    };
-    for (int64_t bi = nbr - 1; bi >= 0; --bi)
-    {
-      const int64_t r0 = r_lo + bi * R;
-      const int64_t rn = std::min<int64_t>(R, r_hi - r0);
-      const int64_t rmax = block_max(m_rows, r0, rn);
-      const int64_t bj_end = tri ? bi : nbc - 1;
-      for (int64_t bj = bj_end; bj >= 0; --bj)
-      {
-        const int64_t c0 = c_lo + bj * C;
-        const int64_t cn = std::min<int64_t>(C, c_hi - c0);
-        const bool diag = tri && bi == bj;
-        if (diag && !include_diag && rn < 2)
-          continue;
-        Tile t;
-        t.r0 = static_cast<int32_t>(r0);
-        t.c0 = static_cast<int32_t>(c0);
-        t.nr = static_cast<int16_t>(rn);
-        t.nc = static_cast<int16_t>(cn);
-        t.prob = prob;
-        t.flags = diag ? TILE_DIAG : 0;
-        if (rmax + block_max(m_cols, c0, cn) - 1 >= fold_threshold)
-          t.flags |= TILE_FOLD;
-        tiles.push_back(t);
-      }
-    }
+    if (tri && (r_lo != c_lo || r_hi != c_hi))
+      fail(SBAMD_ERR_INVALID, "a triangular tile region must be square");
+    tile_grid(m_rows, m_cols, r_lo, r_hi, c_lo, c_hi, R, C, tri, include_diag, prob, fold_threshold, tiles);
   }
 
   // Reorders tiles [b0, b1) of a launch with one workgroup per tile so that

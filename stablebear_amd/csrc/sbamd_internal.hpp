@@ -327,8 +327,8 @@ namespace sbamd
     int32_t out_kind = SBAMD_OUT_DISTMAT;
     int32_t precision = SBAMD_PREC_FP64;
     int32_t tt_f64 = 1, tv_f64 = 1;
-    // default tile: measured best on MI355X for m ~ 50..500 (the autotuner
-    // may pick another)
+    // tile shape (set_default_shape gives a new plan's; the autotuner may
+    // pick another)
     int32_t R = 16, C = 16, cap = 48;
     int32_t ppl = 1; // pairs per lane of the tile kernel
     // per length class shapes (set by the autotuner): the tiles of class k

@@ -30,13 +30,30 @@ def _run(plan, f64):
 
 @pytest.mark.parametrize("shape", [(16, 16, 24, 1), (16, 16, 48, 1), (8, 8, 32, 1), (16, 16, 2, 1),
                                    (4, 4, 7, 1), (32, 32, 48, 2), (32, 32, 64, 2), (16, 16, 24, 2),
-                                   (8, 8, 32, 2), (16, 16, 2, 2), (4, 4, 7, 2), (5, 5, 9, 2)])
+                                   (8, 8, 32, 2), (16, 16, 2, 2), (4, 4, 7, 2), (5, 5, 9, 2),
+                                   (32, 16, 64, 1), (16, 32, 48, 1), (8, 32, 16, 1), (32, 8, 7, 1),
+                                   (24, 16, 9, 1), (32, 16, 24, 2), (16, 8, 5, 2), (5, 9, 3, 1)])
 def test_tile_shapes_all_bit_exact(gpu, shape):
+    """Square and rectangular tiles of the pdist triangle (rectangular tiles
+    that reach the diagonal skip the pairs on or above it)."""
     off, t, v = workloads.c2(90, seed=3)
     exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_PDIST, off, t, v)
     ds, pl = _plan(off, t, v, shape=shape)
     assert pl.shape == shape[:3] and pl.lane_pairs == shape[3]
     np.testing.assert_array_equal(_run(pl, True), exp)
+
+
+@pytest.mark.parametrize("shape", [(32, 16, 64, 1), (16, 32, 48, 1), (7, 24, 5, 1), (16, 8, 9, 2)])
+def test_rectangular_tiles_of_the_kernel_matrix_bit_exact(gpu, shape):
+    """The kernel matrix (diagonal included) with rectangular tiles, FP64
+    against the oracle bit for bit, and FP32C within 1e-6 normwise."""
+    off, t, v = workloads.c2(77, seed=13)
+    exp = oracle.pairwise(oracle.OP_DOT, 1.0, oracle.KIND_KERNEL, off, t, v)
+    ds, pl = _plan(off, t, v, op="dot", shape=shape)
+    np.testing.assert_array_equal(_run(pl, True), exp)
+    ds32, pl32 = _plan(off, t, v, prec="fp32c", op="dot", shape=shape)
+    got = _run(pl32, False).astype(np.float64)
+    assert np.linalg.norm(got - exp) <= 1e-6 * np.linalg.norm(exp)
 
 
 def test_persistent_launch_matches(gpu):

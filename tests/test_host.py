@@ -128,18 +128,25 @@ def _tiles(m_rows, m_cols, R, C):
     return tiles, perm
 
 
-@pytest.mark.parametrize("n,R", [(1, 16), (2, 16), (17, 16), (33, 8), (100, 16)])
-def test_symmetric_tiles_cover_lower_triangle_once(n, R):
+@pytest.mark.parametrize("n,R,C", [(1, 16, 16), (2, 16, 16), (17, 16, 16), (33, 8, 8), (100, 16, 16),
+                                   (100, 32, 16), (100, 16, 32), (70, 24, 16), (33, 32, 16), (2, 32, 16),
+                                   (47, 8, 32)])
+def test_symmetric_tiles_cover_lower_triangle_once(n, R, C):
+    """Square or rectangular tiles of the strict lower triangle: every wanted
+    pair in exactly one tile; a tile without the diagonal flag holds only
+    wanted pairs, and every flagged tile holds an unwanted one."""
     m = np.random.default_rng(n).integers(1, 50, size=n)
-    tiles, perm = _tiles(m, None, R, R)
+    tiles, perm = _tiles(m, None, R, C)
     assert np.all(np.diff(m[perm]) >= 0)  # sorted by record count
     seen = np.zeros((n, n), dtype=int)
-    for r0, c0, nr, nc in tiles:
-        for a in range(r0, r0 + nr):
-            for b in range(c0, c0 + nc):
-                if r0 == c0 and not a > b:
-                    continue
-                seen[a, b] += 1
+    for r0, c0, nr, ncf in tiles:
+        nc, diag = ncf & 0xFFFF, bool(ncf >> 16)
+        assert 1 <= nr <= R and 1 <= nc <= C and r0 % R == 0 and c0 % C == 0
+        a, b = np.meshgrid(np.arange(r0, r0 + nr), np.arange(c0, c0 + nc), indexing="ij")
+        wanted = a > b
+        assert wanted.any()
+        assert diag == (not wanted.all())
+        seen[a[wanted], b[wanted]] += 1
     assert np.array_equal(seen, np.tril(np.ones((n, n), dtype=int), -1))
 
 
