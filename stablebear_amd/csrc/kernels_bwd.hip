@@ -30,10 +30,9 @@
 // each one with t_e < t_b adds w d to X and w d t_e to Y, so that at the end
 // X = H_i(t_b^-) and S_i(t_b) = t_b X - Y. The walk reads the events through
 // the scalar cache (they are the same for every lane), four at a time with
-// the next four in flight, and takes the lane's w_ij from a vector-register
-// array indexed by the event's partner; the widest array the compiler
-// indexes with a scalar is 32 registers, so a chunk's partners come as two
-// sublists (two event lists) of 32 (FP64 weights: 16). The same events then
+// the next four in flight, and reads the lane's w_ij for the four events'
+// partners from the workgroup's weights in LDS. A chunk's partners come as
+// two sublists (two event lists) of 32 (FP64 weights: 16). The same events then
 // move the partners' state on to the next block's T0: for each partner, the
 // last of its events before that time carries its value and P.
 //
@@ -235,13 +234,11 @@ namespace sbamd
     struct WalkVec<float>
     {
       static constexpr int VW = 32;
-      typedef float type __attribute__((ext_vector_type(32)));
     };
     template <>
     struct WalkVec<double>
     {
       static constexpr int VW = 16;
-      typedef double type __attribute__((ext_vector_type(16)));
     };
 
     // Walks the partner events of one sublist from index e while their time
@@ -253,8 +250,8 @@ namespace sbamd
     // arrive while the current four are applied.
     template <typename WT>
     __device__ __forceinline__ int32_t walk_window(const BwdEvent* cl, int32_t n, int32_t e, uint64_t lim,
-                                                   const typename WalkVec<WT>::type& wv, double tb,
-                                                   double& X, double& Y, uint64_t& tie)
+                                                   const WT* w_row, double tb, double& X, double& Y,
+                                                   uint64_t& tie)
     {
       constexpr int VW = WalkVec<WT>::VW;
       if (e >= n)
@@ -264,6 +261,13 @@ namespace sbamd
       for (int u = 0; u < 4; ++u)
         A[u] = walk_load(cl + e, u);
       auto half = [&](WalkEv(&cur)[4], WalkEv(&nxt)[4]) -> bool {
+        // the lane's weights for the four events' partners, requested
+        // before they are needed (an event past the list reads a weight of
+        // the sublist too, and is not applied)
+        WT w[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          w[u] = w_row[(cur[u].col & (VW - 1)) * 64];
         // events are in time order: all four are in the window when the
         // fourth is (and the list has four more), else count them
         int32_t k;
@@ -285,7 +289,7 @@ namespace sbamd
 #pragma unroll
         for (int u = 0; u < 4; ++u)
           if (u < k)
-            walk_step(X, Y, tie, tb, wv[cur[u].col & (VW - 1)], cur[u].t, cur[u].d, cur[u].dt);
+            walk_step(X, Y, tie, tb, w[u], cur[u].t, cur[u].d, cur[u].dt);
         e += k;
         return k == 4;
       };
@@ -297,8 +301,10 @@ namespace sbamd
   }
 
   // Wavefronts per backward workgroup (slices of one task's breakpoints):
-  // with 91 vector registers a lane, two 8-wave workgroups share a CU, so
-  // one's set-up (weights, searches) overlaps the other's walk.
+  // with 57 vector registers a lane, three 8-wave workgroups share a CU (set
+  // by LDS), so one's set-up (weights, searches) overlaps the others' walks.
+  // (With the weights in vector registers instead of LDS, 91 registers and
+  // two workgroups per CU: 1.39-1.42 ms against 1.32-1.36 on 4096 PCFs.)
   constexpr int BWD_WAVES = 8;
 
   // Chunks of partners per workgroup: a breakpoint's two atomics are paid
@@ -466,7 +472,6 @@ namespace sbamd
     constexpr int VW = WalkVec<WT>::VW;
     constexpr int CH = 2 * VW;
     constexpr int G = BWD_CHUNKS;
-    using WV = typename WalkVec<WT>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     // s_w[g][jj * 64 + ii] = w(row ii, partner jj of chunk g)
     WT* const s_w = reinterpret_cast<WT*>(smem);
@@ -644,24 +649,15 @@ namespace sbamd
         int32_t cn0, cn1;
         list_of(g, 0, cl0, cn0);
         list_of(g, 1, cl1, cn1);
-        // partners at the block start, and the lane's row weights (two
-        // named vectors: an array of them would not stay in registers)
-        WV wv0, wv1;
+        // partners at the block start; the lane's row of weights of each
+        // sublist (partner c at [64 c]), read from LDS event by event
+        const WT* const w_row0 = w_g + ib;
+        const WT* const w_row1 = w_g + VW * 64 + ib;
 #pragma unroll
-        for (int v = 0; v < VW; ++v)
+        for (int v = 0; v < CH; ++v)
         {
           const WT w = w_g[v * 64 + ib];
-          wv0[v] = w;
           const double2 fp = fp_g[v];
-          X = fma(static_cast<double>(w), fp.x, X);
-          Y = fma(-static_cast<double>(w), fp.y, Y);
-        }
-#pragma unroll
-        for (int v = 0; v < VW; ++v)
-        {
-          const WT w = w_g[(VW + v) * 64 + ib];
-          wv1[v] = w;
-          const double2 fp = fp_g[VW + v];
           X = fma(static_cast<double>(w), fp.x, X);
           Y = fma(-static_cast<double>(w), fp.y, Y);
         }
@@ -669,8 +665,8 @@ namespace sbamd
         uint64_t tie = 0;
         const int32_t ws0 = __builtin_amdgcn_readlane(ecv, 2 * g);
         const int32_t ws1 = __builtin_amdgcn_readlane(ecv, 2 * g + 1);
-        const int32_t we0 = walk_window<WT>(cl0, cn0, ws0, lim, wv0, tb, X, Y, tie);
-        const int32_t we1 = walk_window<WT>(cl1, cn1, ws1, lim, wv1, tb, X, Y, tie);
+        const int32_t we0 = walk_window<WT>(cl0, cn0, ws0, lim, w_row0, tb, X, Y, tie);
+        const int32_t we1 = walk_window<WT>(cl1, cn1, ws1, lim, w_row1, tb, X, Y, tie);
         ecv = lane == 2 * g ? we0 : (lane == 2 * g + 1 ? we1 : ecv);
 
         if (tie != 0 || edge)
@@ -682,7 +678,7 @@ namespace sbamd
             if (e >= we0 && !(edge && ev.t == Tend))
               break;
             if (ev.t == tb)
-              J = fma(static_cast<double>(wv0[ev.col & (VW - 1)]), ev.d, J);
+              J = fma(static_cast<double>(w_row0[(ev.col & (VW - 1)) * 64]), ev.d, J);
           }
           for (int32_t e = ws1; e < cn1; ++e)
           {
@@ -690,7 +686,7 @@ namespace sbamd
             if (e >= we1 && !(edge && ev.t == Tend))
               break;
             if (ev.t == tb)
-              J = fma(static_cast<double>(wv1[ev.col & (VW - 1)]), ev.d, J);
+              J = fma(static_cast<double>(w_row1[(ev.col & (VW - 1)) * 64]), ev.d, J);
           }
         }
 
