@@ -940,15 +940,15 @@ namespace sbamd
     // dense matrix; the host-to-host jobs also make cross plans that write
     // compressed outputs (pairs whose row index exceeds every column index).
     // out_base is added to every output index.
-    // Default tile: 48 slots of 64 staged records (three workgroups of 8
-    // wavefronts per CU, set by LDS): 16 rows x 32 columns for a pdist /
-    // kernel plan, 32 x 16 for a cdist (measured best on MI355X for m ~
-    // 50..500: fewer phases per tile than 16 x 16 x 48 at the same
-    // occupancy); 16 x 16 x 48 where a workgroup cannot hold that much LDS.
+    // Default tile: 16 rows x 32 columns, 64 staged records per slot (48
+    // slots: three workgroups of 8 wavefronts per CU, set by LDS). Measured
+    // on MI355X: fewer phases per tile than 16 x 16 x 48 at the same
+    // occupancy, for pdists (m ~ 50..500) and long cdists (m ~ 500..5000)
+    // alike; 16 x 16 x 48 where a workgroup cannot hold that much LDS.
     void set_default_shape(Plan& pl)
     {
-      pl.R = pl.cdist ? 32 : 16;
-      pl.C = pl.cdist ? 16 : 32;
+      pl.R = 16;
+      pl.C = 32;
       pl.cap = 64;
       pl.ppl = 1;
       if (fwd_lds_bytes(pl.tt_f64, pl.tv_f64, pl.R, pl.C, pl.cap) > pl.ctx->lds_per_block)
