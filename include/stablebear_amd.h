@@ -325,6 +325,16 @@ int sbamd_plan_set_shape(sbamd_plan* plan, const sbamd_tile_shape* shape);
 int sbamd_pairwise_submit(sbamd_ctx* ctx, const sbamd_pcf_set* rows, const sbamd_pcf_set* cols,
                           int32_t op, double p, int32_t out_kind, int32_t precision, void* out_host,
                           int32_t out_dtype, sbamd_job** out);
+/* The same job from strided views (sbamd_pcf_view: AoS {t, v} points, as a
+ * list of (m, 2) point arrays concatenates to, need no split into time and
+ * value arrays), host or device. The offsets are checked at submission; the
+ * points by the GPU packer during the job, whose wait then returns the first
+ * offending PCF's error. Serves the same reference factories as
+ * sbamd_pairwise_submit (cuda_matrix_integrate_api.hpp:19-113), whose
+ * Python callers hand over lists of PCFs. */
+int sbamd_pairwise_submit_view(sbamd_ctx* ctx, const sbamd_pcf_view* rows, const sbamd_pcf_view* cols,
+                               int32_t op, double p, int32_t out_kind, int32_t precision, void* out_host,
+                               int32_t out_dtype, sbamd_job** out);
 /* SBAMD_OK when finished, SBAMD_TIMEOUT if still running after timeout_ms
  * (< 0 waits forever); the job's own error status once finished. */
 int sbamd_job_wait(sbamd_job* job, int64_t timeout_ms);

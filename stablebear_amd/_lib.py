@@ -142,6 +142,9 @@ _SIGS = {
     "sbamd_pairwise_submit": (None, [vp, ctypes.POINTER(PcfSet), ctypes.POINTER(PcfSet),
                                      ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
                                      ctypes.c_int32, vp, ctypes.c_int32, ctypes.POINTER(vp)]),
+    "sbamd_pairwise_submit_view": (None, [vp, ctypes.POINTER(PcfView), ctypes.POINTER(PcfView),
+                                          ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
+                                          ctypes.c_int32, vp, ctypes.c_int32, ctypes.POINTER(vp)]),
     "sbamd_job_wait": (None, [vp, ctypes.c_int64]),
     "sbamd_job_progress": (None, [vp, c_u64p, c_u64p]),
     "sbamd_job_cancel": (None, [vp]),

@@ -1722,13 +1722,46 @@ namespace
     ev_free();
   }
 
-  void run_job(Ctx* ctx, sbamd_job* jp, const sbamd_pcf_set& rows, const sbamd_pcf_set* cols, int32_t op,
+  // A job's set from a view: small contiguous host sets through the host
+  // packer, the rest through the GPU packer (which also checks the points
+  // and reports the first offending PCF with the host rules' message).
+  void make_set_view(Ctx* ctx, const sbamd_pcf_view& in, int32_t precision, Set& s)
+  {
+    const int64_t es = in.dtype == SBAMD_F64 ? 8 : 4;
+    if (in.location == SBAMD_HOST && in.t_stride == es && in.v_stride == es)
+    {
+      sbamd_pcf_set set;
+      set.n = in.n;
+      set.offsets = in.offsets;
+      set.t = in.t;
+      set.v = in.v;
+      set.dtype = in.dtype;
+      make_set_auto(ctx, set, precision, s);
+      return;
+    }
+    make_set_gpu(ctx, in, precision, s);
+  }
+
+  sbamd_pcf_view view_of(const sbamd_pcf_set& in)
+  {
+    sbamd_pcf_view v;
+    v.n = in.n;
+    v.offsets = in.offsets;
+    v.t = in.t;
+    v.v = in.v;
+    v.t_stride = v.v_stride = in.dtype == SBAMD_F64 ? 8 : 4;
+    v.dtype = in.dtype;
+    v.location = SBAMD_HOST;
+    return v;
+  }
+
+  void run_job(Ctx* ctx, sbamd_job* jp, const sbamd_pcf_view& rows, const sbamd_pcf_view* cols, int32_t op,
                double p, int32_t out_kind, int32_t precision, void* out_host, int32_t out_dtype)
   {
     Set rs, cs;
-    make_set_auto(ctx, rows, precision, rs);
+    make_set_view(ctx, rows, precision, rs);
     if (cols)
-      make_set_auto(ctx, *cols, precision, cs);
+      make_set_view(ctx, *cols, precision, cs);
     const size_t eb = out_dtype == SBAMD_F64 ? 8 : 4;
     const uint64_t nr = static_cast<uint64_t>(rs.n);
     const uint64_t out_count = cols ? nr * static_cast<uint64_t>(cs.n)
@@ -1749,6 +1782,70 @@ namespace
     // JOB_BUFFERS band buffers within the budget
     const uint64_t chunk = std::max<uint64_t>(budget / (JOB_BUFFERS * eb), 1);
     job_bands(ctx, jp, rs, cols ? &cs : nullptr, op, p, out_kind, out_host, out_dtype, chunk);
+  }
+}
+
+namespace sbamd
+{
+  namespace
+  {
+    // The job thread of sbamd_pairwise_submit(_view); the views' memory stays
+    // the caller's until the job ends.
+    void submit_job(sbamd_ctx* ctx, const sbamd_pcf_view& rows, const sbamd_pcf_view* cols, int32_t op, double p,
+                    int32_t out_kind, int32_t precision, void* out_host, int32_t out_dtype, sbamd_job** out)
+    {
+      need(ctx, "ctx");
+      need(out, "out");
+      *out = nullptr;
+      check_out_dtype(out_dtype);
+      if (op == SBAMD_OP_LP && !(p >= 1.0))
+        fail(SBAMD_ERR_INVALID, "p must be >= 1.");
+      (void)kernel_op(op, p);
+      if (rows.n < 0 || (cols && cols->n < 0))
+        fail(SBAMD_ERR_INVALID, "PCF set size must be non-negative");
+      if (rows.n > 0 || (cols && cols->n > 0))
+        need(out_host, "out_host");
+      auto job = std::make_unique<sbamd_job>();
+      sbamd_job* jp = job.get();
+      const sbamd_pcf_view rcopy = rows;
+      const bool has_cols = cols != nullptr;
+      const sbamd_pcf_view ccopy = cols ? *cols : sbamd_pcf_view{};
+      uint64_t total = 0;
+      if (has_cols)
+        total = static_cast<uint64_t>(rows.n) * static_cast<uint64_t>(cols->n);
+      else if (out_kind == SBAMD_OUT_SYMMAT)
+        total = static_cast<uint64_t>(rows.n) * (rows.n + 1) / 2;
+      else
+        total = rows.n > 1 ? tri(static_cast<uint64_t>(rows.n)) : 0;
+      jp->total = total;
+      jp->worker = std::thread([=]() {
+        int status = SBAMD_OK;
+        std::string err;
+        try
+        {
+          SBAMD_HIP(hipSetDevice(ctx->device));
+          if (total > 0)
+            run_job(ctx, jp, rcopy, has_cols ? &ccopy : nullptr, op, p, out_kind, precision, out_host,
+                    out_dtype);
+        }
+        catch (const Error& e)
+        {
+          status = e.code;
+          err = e.msg;
+        }
+        catch (const std::exception& e)
+        {
+          status = SBAMD_ERR_RUNTIME;
+          err = e.what();
+        }
+        std::lock_guard<std::mutex> lk(jp->mu);
+        jp->status = status;
+        jp->error = err;
+        jp->done = true;
+        jp->cv.notify_all();
+      });
+      *out = job.release();
+    }
   }
 }
 
@@ -2752,62 +2849,36 @@ extern "C"
                             int32_t out_dtype, sbamd_job** out)
   {
     return guarded([&] {
-      need(ctx, "ctx");
       need(rows, "rows");
-      need(out, "out");
-      *out = nullptr;
-      check_out_dtype(out_dtype);
-      if (op == SBAMD_OP_LP && !(p >= 1.0))
-        fail(SBAMD_ERR_INVALID, "p must be >= 1.");
-      (void)kernel_op(op, p);
       // Validate on the caller's thread so input errors surface at submit.
       validate_set(*rows);
       if (cols)
         validate_set(*cols);
-      if (rows->n > 0 || (cols && cols->n > 0))
-        need(out_host, "out_host");
-      auto job = std::make_unique<sbamd_job>();
-      sbamd_job* jp = job.get();
-      const sbamd_pcf_set rcopy = *rows;
-      const bool has_cols = cols != nullptr;
-      const sbamd_pcf_set ccopy = cols ? *cols : sbamd_pcf_set{};
-      uint64_t total = 0;
-      if (has_cols)
-        total = static_cast<uint64_t>(rows->n) * static_cast<uint64_t>(cols->n);
-      else if (out_kind == SBAMD_OUT_SYMMAT)
-        total = static_cast<uint64_t>(rows->n) * (rows->n + 1) / 2;
-      else
-        total = rows->n > 1 ? tri(static_cast<uint64_t>(rows->n)) : 0;
-      jp->total = total;
-      jp->worker = std::thread([=]() {
-        int status = SBAMD_OK;
-        std::string err;
-        try
-        {
-          SBAMD_HIP(hipSetDevice(ctx->device));
-          if (total > 0)
-            run_job(ctx, jp, rcopy, has_cols ? &ccopy : nullptr, op, p, out_kind, precision, out_host,
-                    out_dtype);
-        }
-        catch (const Error& e)
-        {
-          status = e.code;
-          err = e.msg;
-        }
-        catch (const std::exception& e)
-        {
-          status = SBAMD_ERR_RUNTIME;
-          err = e.what();
-        }
-        std::lock_guard<std::mutex> lk(jp->mu);
-        jp->status = status;
-        jp->error = err;
-        jp->done = true;
-        jp->cv.notify_all();
-      });
-      *out = job.release();
+      const sbamd_pcf_view rv = view_of(*rows);
+      const sbamd_pcf_view cv = cols ? view_of(*cols) : sbamd_pcf_view{};
+      submit_job(ctx, rv, cols ? &cv : nullptr, op, p, out_kind, precision, out_host, out_dtype, out);
     });
   }
+
+  int sbamd_pairwise_submit_view(sbamd_ctx* ctx, const sbamd_pcf_view* rows, const sbamd_pcf_view* cols,
+                                 int32_t op, double p, int32_t out_kind, int32_t precision, void* out_host,
+                                 int32_t out_dtype, sbamd_job** out)
+  {
+    return guarded([&] {
+      need(rows, "rows");
+      // the offsets here; the points on the device, during the job
+      for (const sbamd_pcf_view* v : {rows, cols})
+        if (v)
+        {
+          if (v->location != SBAMD_HOST && v->location != SBAMD_DEVICE)
+            fail(SBAMD_ERR_INVALID, "PCF view location must be SBAMD_HOST or SBAMD_DEVICE");
+          if (v->location == SBAMD_HOST)
+            check_set_offsets(v->n, v->offsets, v->dtype);
+        }
+      submit_job(ctx, *rows, cols, op, p, out_kind, precision, out_host, out_dtype, out);
+    });
+  }
+
 
   int sbamd_job_wait(sbamd_job* job, int64_t timeout_ms)
   {

@@ -56,7 +56,16 @@ class Csr:
     def n(self) -> int:
         return len(self.offsets) - 1
 
+    @property
+    def strided(self) -> bool:
+        """True when t and v are strided views (of AoS points)."""
+        return not (self.t.flags["C_CONTIGUOUS"] and self.v.flags["C_CONTIGUOUS"])
+
     def struct(self) -> L.PcfSet:
+        """The C struct of contiguous arrays (strided t / v are copied once)."""
+        if self.strided:
+            self.t = np.ascontiguousarray(self.t)
+            self.v = np.ascontiguousarray(self.v)
         s = L.PcfSet()
         s.n = self.n
         s.offsets = self.offsets.ctypes.data_as(L.c_i64p)
@@ -91,16 +100,18 @@ def csr_from_arrays(offsets, t, v) -> Csr:
 
 
 def csr_from_pcfs(pcfs, dtype) -> Csr:
-    """CSR set from a list of Pcf of one dtype (pcf32 or pcf64)."""
+    """CSR set from a list of Pcf of one dtype (pcf32 or pcf64). The points
+    are concatenated once; t and v are strided views of them (AoS), which the
+    jobs take as they are and other callers copy when they need flat arrays."""
     np_dt = np.float64 if dtype is pcf64 else np.float32
     n = len(pcfs)
     lens = np.fromiter((p._arr.shape[0] for p in pcfs), dtype=np.int64, count=n)
     offsets = np.zeros(n + 1, dtype=np.int64)
     np.cumsum(lens, out=offsets[1:])
     if n:
-        pts = np.concatenate([p._arr for p in pcfs], axis=0).astype(np_dt, copy=False)
-        t = np.ascontiguousarray(pts[:, 0])
-        v = np.ascontiguousarray(pts[:, 1])
+        pts = np.ascontiguousarray(np.concatenate([p._arr for p in pcfs], axis=0), dtype=np_dt)
+        t = pts[:, 0]
+        v = pts[:, 1]
     else:
         t = np.zeros(0, dtype=np_dt)
         v = np.zeros(0, dtype=np_dt)
@@ -211,16 +222,26 @@ def autotune_cache_stats():
 
 def submit(rows: Csr, cols: Csr | None, op: int, p: float, out_kind: int, precision: int,
            out: np.ndarray, desc: str, total: int) -> Task:
-    """Start a host-to-host pairwise job writing into ``out`` (kept alive by the task)."""
-    rs = rows.struct()
-    cs = cols.struct() if cols is not None else None
+    """Start a host-to-host pairwise job writing into ``out`` (kept alive by
+    the task). Sets of strided (AoS) points go in as views, unsplit; their
+    points are checked on the device during the job."""
     job = ctypes.c_void_p()
     out_dtype = L.F64 if out.dtype == np.float64 else L.F32
-    rc = L.lib().sbamd_pairwise_submit(context(), ctypes.byref(rs),
-                                       ctypes.byref(cs) if cs is not None else None, op,
-                                       float(p), out_kind, precision,
-                                       out.ctypes.data if out.size else None, out_dtype,
-                                       ctypes.byref(job))
+    out_ptr = out.ctypes.data if out.size else None
+    if rows.strided or (cols is not None and cols.strided):
+        rs = csr_view(rows)
+        cs = csr_view(cols) if cols is not None else None
+        rc = L.lib().sbamd_pairwise_submit_view(context(), ctypes.byref(rs),
+                                                ctypes.byref(cs) if cs is not None else None, op,
+                                                float(p), out_kind, precision, out_ptr, out_dtype,
+                                                ctypes.byref(job))
+    else:
+        rs = rows.struct()
+        cs = cols.struct() if cols is not None else None
+        rc = L.lib().sbamd_pairwise_submit(context(), ctypes.byref(rs),
+                                           ctypes.byref(cs) if cs is not None else None, op,
+                                           float(p), out_kind, precision, out_ptr, out_dtype,
+                                           ctypes.byref(job))
     L.check(rc)
     return Task(job, (rows, cols, rs, cs, out), desc, total)
 

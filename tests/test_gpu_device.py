@@ -318,6 +318,65 @@ def test_job_bands_under_small_output_budget(gpu, kind):
         runtime.set_output_budget(0)
 
 
+def _aos(off, t, v):
+    """A CSR set whose times and values are strided views of (m, 2) points,
+    as a list of Pcf arrays concatenates to."""
+    pts = np.ascontiguousarray(np.stack([t, v], axis=1))
+    return runtime.Csr(np.asarray(off, dtype=np.int64), pts[:, 0], pts[:, 1], L.F64)
+
+
+@pytest.mark.parametrize("kind,budget", [("pdist", 0), ("kernel", 0), ("cdist", 0), ("pdist", 9), ("cdist", 9)])
+def test_job_from_strided_points_matches(gpu, kind, budget):
+    """Jobs handed AoS points as strided views (sbamd_pairwise_submit_view)
+    give the results of the contiguous-array jobs, bit for bit in FP64, with
+    one plan or in bands (a small output budget)."""
+    n = 700
+    off, t, v = workloads.c2(n, seed=44)
+    rows = _aos(off, t, v)
+    assert rows.strided
+    cols = None
+    if kind == "cdist":
+        yo, yt, yv = workloads.c2(300, seed=45)
+        cols = _aos(yo, yt, yv)
+        total, op, out_kind = n * 300, L.OP_L1, L.OUT_DENSE
+        exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_CDIST, off, t, v, yo, yt, yv)
+    elif kind == "kernel":
+        total, op, out_kind = n * (n + 1) // 2, L.OP_DOT, L.OUT_SYMMAT
+        exp = oracle.pairwise(oracle.OP_DOT, 1.0, oracle.KIND_KERNEL, off, t, v)
+    else:
+        total, op, out_kind = n * (n - 1) // 2, L.OP_L1, L.OUT_DISTMAT
+        exp = oracle.pairwise(oracle.OP_L1, 1.0, oracle.KIND_PDIST, off, t, v)
+    try:
+        runtime.set_output_budget(total * 8 // budget if budget else 0)
+        out = np.full(total, np.nan)
+        task = runtime.submit(rows, cols, op, 1.0, out_kind, L.PREC_FP64, out, "d", total)
+        task.wait()
+        np.testing.assert_array_equal(out, exp)
+    finally:
+        runtime.set_output_budget(0)
+
+
+@pytest.mark.parametrize("bad,msg", [("order", "non-decreasing"), ("start", "t=0"), ("inf", "finite")])
+def test_job_from_strided_points_reports_bad_points(gpu, bad, msg):
+    """A strided job checks the points on the device: the wait reports the
+    first offending PCF with the host rules' message."""
+    off, t, v = workloads.c2(400, seed=46)
+    t = t.copy()
+    i = 123
+    if bad == "order":
+        t[off[i] + 3] = t[off[i] + 1]
+    elif bad == "start":
+        t[off[i]] = 1e-9
+    else:
+        t[off[i] + 2] = np.inf
+    rows = _aos(off, t, v)
+    total = 400 * 399 // 2
+    out = np.zeros(total)
+    task = runtime.submit(rows, None, L.OP_L1, 1.0, L.OUT_DISTMAT, L.PREC_FP64, out, "d", total)
+    with pytest.raises(ValueError, match=msg):
+        task.wait()
+
+
 def test_device_set_value_update(gpu):
     off, t, v = workloads.c2(80, seed=12)
     ds, pl = _plan(off, t, v)
