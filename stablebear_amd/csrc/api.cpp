@@ -839,6 +839,7 @@ namespace sbamd
       double* q;
       double* a;
       double* w;
+      int32_t* ties; // cross-tie flag of the lists (NULL: not kept, ties assumed)
     };
 
     int backward_checks(const Plan* plan, const void* grad_out_device, const void* fwd_out_device,
@@ -919,6 +920,7 @@ namespace sbamd
       B.wsum = wk.w;
       B.grad_v = grad_v_device;
       B.grad_t = grad_t_device;
+      B.cross_ties = wk.ties;
       launch_backward(B, lists_ready, st);
     }
 
@@ -2496,6 +2498,7 @@ extern "C"
         plan->bwd_q = DeviceBuffer(M * 8);
         plan->bwd_a = DeviceBuffer(M * 8);
         plan->bwd_w = DeviceBuffer(N * 8);
+        plan->bwd_ties = DeviceBuffer(64);
         plan->bwd_lists_ready = false;
       }
       BwdWork wk;
@@ -2513,6 +2516,7 @@ extern "C"
       wk.q = plan->bwd_q.as<double>();
       wk.a = plan->bwd_a.as<double>();
       wk.w = plan->bwd_w.as<double>();
+      wk.ties = plan->bwd_ties.as<int32_t>();
       run_backward(*plan, kop, wk, plan->bwd_lists_ready, grad_out_device, fwd_out_device, out_dtype,
                    grad_v_device, grad_t_device, grad_dtype, pick_stream(plan->ctx, stream));
       plan->bwd_lists_ready = true;
@@ -2551,6 +2555,7 @@ extern "C"
       wk.ids[0] = reinterpret_cast<int32_t*>(base + NOALLOC_IDS0);
       wk.ids[1] = reinterpret_cast<int32_t*>(base + NOALLOC_IDS1);
       wk.w = reinterpret_cast<double*>(base + NOALLOC_W);
+      wk.ties = nullptr; // the lists are sorted on every call here, without the global order that rules ties out
       // the working array is shared by every plan on the device: sort every time
       run_backward(*plan, kop, wk, false, grad_out_device, fwd_out_device, out_dtype, grad_v_device,
                    grad_t_device, grad_dtype, pick_stream(plan->ctx, stream));

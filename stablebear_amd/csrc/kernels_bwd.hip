@@ -228,6 +228,33 @@ namespace sbamd
       tie |= __ballot(t == tb);
     }
 
+    // The same without the tie flags, for lists in which no two PCFs share a
+    // breakpoint time (the only ties left are a PCF's with itself, which the
+    // caller adds on its own).
+    __device__ __forceinline__ void walk_step_untied(double& X, double& Y, double tb, float w, double t, double d,
+                                                     double dt)
+    {
+      float wm;
+      double wd;
+      asm volatile("v_cmp_lt_f64 vcc, %[t], %[tb]\n\t"
+                   "s_nop 1\n\t"
+                   "v_cndmask_b32 %[wm], 0, %[w], vcc\n\t"
+                   "v_cvt_f64_f32 %[wd], %[wm]\n\t"
+                   "v_fmac_f64 %[X], %[d], %[wd]\n\t"
+                   "v_fmac_f64 %[Y], %[dt], %[wd]"
+                   : [X] "+v"(X), [Y] "+v"(Y), [wm] "=&v"(wm), [wd] "=&v"(wd)
+                   : [t] "s"(t), [tb] "v"(tb), [w] "v"(w), [d] "s"(d), [dt] "s"(dt)
+                   : "vcc");
+    }
+
+    __device__ __forceinline__ void walk_step_untied(double& X, double& Y, double tb, double w, double t, double d,
+                                                     double dt)
+    {
+      const double wm = t < tb ? w : 0.0;
+      X = fma(wm, d, X);
+      Y = fma(wm, dt, Y);
+    }
+
     template <typename WT>
     struct WalkVec;
     template <>
@@ -248,7 +275,7 @@ namespace sbamd
     // padding, so a group may run past the list); the next four are
     // requested after the current ones have been waited for, so that they
     // arrive while the current four are applied.
-    template <typename WT>
+    template <typename WT, bool TIES>
     __device__ __forceinline__ int32_t walk_window(const BwdEvent* cl, int32_t n, int32_t e, uint64_t lim,
                                                    const WT* w_row, double tb, double& X, double& Y,
                                                    uint64_t& tie)
@@ -289,7 +316,12 @@ namespace sbamd
 #pragma unroll
         for (int u = 0; u < 4; ++u)
           if (u < k)
-            walk_step(X, Y, tie, tb, w[u], cur[u].t, cur[u].d, cur[u].dt);
+          {
+            if constexpr (TIES)
+              walk_step(X, Y, tie, tb, w[u], cur[u].t, cur[u].d, cur[u].dt);
+            else
+              walk_step_untied(X, Y, tb, w[u], cur[u].t, cur[u].d, cur[u].dt);
+          }
         e += k;
         return k == 4;
       };
@@ -492,6 +524,7 @@ namespace sbamd
     if (cr >= ngroups)
       return;
     const int64_t grp = ngroups - 1 - cr;
+    const bool ties_possible = __builtin_amdgcn_readfirstlane(B.cross_ties ? *B.cross_ties : 1) != 0;
     const int64_t band = nbands - 1 - slot % nbands;
     const int nrow = static_cast<int>(min(static_cast<int64_t>(BWD_GROUP), B.n - band * BWD_GROUP));
 
@@ -635,6 +668,9 @@ namespace sbamd
       lim = (static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(lim >> 32))) << 32) |
             static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(lim)));
       const bool edge = !lastblk && __ballot(valid && tb == Tend) != 0;
+      // without ties between PCFs (and none at this block's end) the walk
+      // skips the tie flags; a block with an edge takes the checked walk
+      const bool untied = !ties_possible && !edge;
 
       double X = 0.0, Y = 0.0, J = 0.0; // over the group's partners
 #pragma unroll 1
@@ -665,8 +701,22 @@ namespace sbamd
         uint64_t tie = 0;
         const int32_t ws0 = __builtin_amdgcn_readlane(ecv, 2 * g);
         const int32_t ws1 = __builtin_amdgcn_readlane(ecv, 2 * g + 1);
-        const int32_t we0 = walk_window<WT>(cl0, cn0, ws0, lim, w_row0, tb, X, Y, tie);
-        const int32_t we1 = walk_window<WT>(cl1, cn1, ws1, lim, w_row1, tb, X, Y, tie);
+        int32_t we0, we1;
+        if (untied)
+        {
+          we0 = walk_window<WT, false>(cl0, cn0, ws0, lim, w_row0, tb, X, Y, tie);
+          we1 = walk_window<WT, false>(cl1, cn1, ws1, lim, w_row1, tb, X, Y, tie);
+          // the one tie left: the lane's own PCF among the partners, whose
+          // event at tb is the lane's breakpoint itself
+          const int64_t js = band * BWD_GROUP + ib - (grp * G + g) * CH;
+          if (js >= 0 && js < CH)
+            J = fma(static_cast<double>(w_g[js * 64 + ib]), E.d, J);
+        }
+        else
+        {
+          we0 = walk_window<WT, true>(cl0, cn0, ws0, lim, w_row0, tb, X, Y, tie);
+          we1 = walk_window<WT, true>(cl1, cn1, ws1, lim, w_row1, tb, X, Y, tie);
+        }
         ecv = lane == 2 * g ? we0 : (lane == 2 * g + 1 ? we1 : ecv);
 
         if (tie != 0 || edge)
@@ -770,6 +820,18 @@ namespace sbamd
     }
   }
 
+  // Thread per position of the whole set's time order: flags a breakpoint
+  // time (> 0) that two different PCFs share.
+  __global__ void bwd_ties_kernel(BwdParams B, int32_t from)
+  {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p + 1 >= B.records)
+      return;
+    const double t = B.keys[from][p];
+    if (t > 0.0 && t == B.keys[from][p + 1] && B.col0[B.ids[from][p]] != B.col0[B.ids[from][p + 1]])
+      atomicOr(B.cross_ties, 1);
+  }
+
   void launch_backward(const BwdParams& prm, bool lists_ready, hipStream_t stream)
   {
     if (prm.records <= 0 || prm.n <= 0)
@@ -802,6 +864,25 @@ namespace sbamd
     else
       hipLaunchKernelGGL(bwd_events_kernel, dim3(rgrid), dim3(256), 0, stream, prm, from, BWD_GROUP, prm.events);
     SBAMD_HIP(hipGetLastError());
+    if (!lists_ready && prm.cross_ties)
+    {
+      // the whole set's time order (merge levels over all PCFs, kept with the
+      // lists' flag), then a check for a breakpoint time that two PCFs share:
+      // without one the gather walk skips its tie flags
+      int32_t all = 1;
+      while (all < prm.n)
+        all *= 2;
+      int f = from;
+      for (int32_t width = BWD_GROUP; width < prm.n; width *= 2)
+      {
+        hipLaunchKernelGGL(bwd_merge_kernel, dim3(rgrid), dim3(256), 0, stream, prm, width, f, all);
+        SBAMD_HIP(hipGetLastError());
+        f ^= 1;
+      }
+      launch_fill_zero(prm.cross_ties, sizeof(int32_t), stream);
+      hipLaunchKernelGGL(bwd_ties_kernel, dim3(rgrid), dim3(256), 0, stream, prm, f);
+      SBAMD_HIP(hipGetLastError());
+    }
     // the sort keys are free now: pst0 takes the other buffer
     BwdParams B = prm;
     B.pst0 = prm.keys[from ^ 1];

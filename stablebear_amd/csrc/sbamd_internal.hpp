@@ -197,6 +197,9 @@ namespace sbamd
     double* wsum; // per PCF: sum of its pair weights
     void* grad_v;
     void* grad_t;
+    // set by the list build: 1 if two different PCFs share a breakpoint time
+    // (> 0), else 0; NULL: not known, every walk checks for ties
+    int32_t* cross_ties;
   };
 
   // ---------------------------------------------------------------------
@@ -359,7 +362,7 @@ namespace sbamd
     DeviceBuffer counter;
     // backward working arrays (kernels_bwd.hip), allocated on first use
     DeviceBuffer bwd_ooff, bwd_key0, bwd_val0, bwd_col0, bwd_src0, bwd_keys[2], bwd_ids[2],
-        bwd_events, bwd_events2, bwd_q, bwd_a, bwd_w;
+        bwd_events, bwd_events2, bwd_q, bwd_a, bwd_w, bwd_ties;
     // the event lists in bwd_events / bwd_events2 are sorted (a set's
     // breakpoint times never change, so they stay sorted until reallocated)
     bool bwd_lists_ready = false;

@@ -58,6 +58,35 @@ def test_gradient_matches_cpu_analytic(gpu, kind, prec):
     assert np.all(gt[~m] == 0.0)
 
 
+@pytest.mark.parametrize("kind", ["pdist", "kernel"])
+def test_gradient_with_times_shared_between_pcfs(gpu, kind):
+    """Breakpoint times on an integer grid, so that many PCFs (in one band
+    and across bands) share times: the backward sees the ties between PCFs
+    and takes the mean of the limits there, as the oracle does. (Sets whose
+    PCFs share no time take the walk without tie flags, checked by the
+    tests above.)"""
+    rng = np.random.default_rng(77)
+    n = 200
+    m = rng.integers(3, 25, size=n)
+    off = np.concatenate([[0], np.cumsum(m)]).astype(np.int64)
+    t = np.concatenate([np.concatenate([[0.0], np.sort(rng.choice(np.arange(1, 60), k - 1, replace=False))])
+                        for k in m]).astype(np.float64)
+    v = rng.standard_normal(off[-1])
+    v[off[1:] - 1] = 0.0
+    if kind == "kernel":
+        W = rng.uniform(0, 1, n * (n + 1) // 2)
+        op, okind = oracle.OP_DOT, oracle.KIND_KERNEL
+    else:
+        W = rng.uniform(0, 1, n * (n - 1) // 2)
+        op, okind = oracle.OP_LP, oracle.KIND_PDIST
+    out, gv, gt = _grads(kind, "fp64", off, t, v, W)
+    fwd = oracle.pairwise(op, 2.0 if op == oracle.OP_LP else 1.0, okind, off, t, v)
+    ev, et = oracle.grad(op, okind, off, t, v, W, fwd if op == oracle.OP_LP else None)
+    assert np.max(np.abs(gv - ev)) <= 1e-9 * np.abs(ev).max()
+    mi = _interior_times(off)
+    assert np.max(np.abs(gt[mi] - et[mi])) <= 1e-9 * np.abs(et).max()
+
+
 def test_gradient_matches_finite_differences(gpu):
     off, t, v, w = _inputs(40, seed=7)
     n = len(off) - 1
