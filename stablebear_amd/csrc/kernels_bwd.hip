@@ -51,7 +51,10 @@
 // mean of H_i's left and right limits there (the mean of the partner's values
 // on both sides). For a self pair (kernel diagonal) that is the exact
 // derivative; for two PCFs jumping together the two one-sided results add up
-// to the derivative of moving the shared time. S is continuous there.
+// to the derivative of moving the shared time. S is continuous there. A
+// plan's first backward also sorts the whole set by time and flags a time
+// that two PCFs share; without one, the walk skips its tie test and each
+// lane adds its own PCF's term (the only tie left) directly.
 #include "walk_common.hpp"
 
 namespace sbamd
