@@ -23,8 +23,7 @@
 // a group of two chunks. Its wavefronts split the band's breakpoints (one
 // time-ordered list) into slices and take them 64 at a time, one breakpoint
 // t_b (of row i_b) per lane. For a block whose first breakpoint is at T0, a
-// lane starts, chunk after chunk, from the partners' state at T0 (f_j and P_j
-// of each partner, in LDS):
+// lane starts from its row's sums over the group's partners at T0,
 //   X = sum_j w_ij f_j(T0),  Y = -sum_j w_ij P_j(T0),
 // and walks the partner breakpoints from T0 to the block's last breakpoint:
 // each one with t_e < t_b adds w d to X and w d t_e to Y, so that at the end
@@ -32,9 +31,12 @@
 // the scalar cache (they are the same for every lane), four at a time with
 // the next four in flight, and reads the lane's w_ij for the four events'
 // partners from the workgroup's weights in LDS. A chunk's partners come as
-// two sublists (two event lists) of 32 (FP64 weights: 16). The same events then
-// move the partners' state on to the next block's T0: for each partner, the
-// last of its events before that time carries its value and P.
+// two sublists (two event lists) of 32 (FP64 weights: 16). The row sums at T0
+// live in the lanes, row r in lane r: formed once per slice from the
+// partners' state at its start (f_j and P_j, in LDS), then moved along by
+// every event of every window (unmasked, with the lane's own row's weight),
+// so that at a block's end they hold the next block's T0 values; a block's
+// lanes read their rows' sums from those lanes.
 //
 // Each breakpoint adds S_i(t_b) (summed over the group's partners) to q at
 // the record it starts and H_i(t_b) to a at the record it ends; the finalize
@@ -280,8 +282,8 @@ namespace sbamd
     // arrive while the current four are applied.
     template <typename WT, bool TIES>
     __device__ __forceinline__ int32_t walk_window(const BwdEvent* cl, int32_t n, int32_t e, uint64_t lim,
-                                                   const WT* w_row, double tb, double& X, double& Y,
-                                                   uint64_t& tie)
+                                                   const WT* w_row, const WT* w_own, double tb, double& X,
+                                                   double& Y, double& XA, double& YA, uint64_t& tie)
     {
       constexpr int VW = WalkVec<WT>::VW;
       if (e >= n)
@@ -294,10 +296,13 @@ namespace sbamd
         // the lane's weights for the four events' partners, requested
         // before they are needed (an event past the list reads a weight of
         // the sublist too, and is not applied)
-        WT w[4];
+        WT w[4], wo[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
+        {
           w[u] = w_row[(cur[u].col & (VW - 1)) * 64];
+          wo[u] = w_own[(cur[u].col & (VW - 1)) * 64];
+        }
         // events are in time order: all four are in the window when the
         // fourth is (and the list has four more), else count them
         int32_t k;
@@ -324,6 +329,10 @@ namespace sbamd
               walk_step(X, Y, tie, tb, w[u], cur[u].t, cur[u].d, cur[u].dt);
             else
               walk_step_untied(X, Y, tb, w[u], cur[u].t, cur[u].d, cur[u].dt);
+            // the lane's own row (row = lane) takes every event of the window
+            const double wd = static_cast<double>(wo[u]);
+            XA = fma(wd, cur[u].d, XA);
+            YA = fma(wd, cur[u].dt, YA);
           }
         e += k;
         return k == 4;
@@ -650,6 +659,26 @@ namespace sbamd
       ecv = lane == k ? e : ecv;
     }
     __builtin_amdgcn_wave_barrier();
+    // Row `lane`'s sums over the group's partners at the current block start,
+    // X = sum_j w f_j and Y = -sum_j w P_j: formed once from the partners'
+    // state at the slice start, then moved along by every event the walk
+    // reads; a block's lanes take their rows' sums from these lanes.
+    double XA = 0.0, YA = 0.0;
+    for (int g = 0; g < G; ++g)
+    {
+      if ((grp * G + g) * CH >= B.n)
+        break;
+      const WT* const w_g = s_w + g * CH * 64;
+      const double2* const fp_g = s_fp + g * CH;
+#pragma unroll 8
+      for (int v = 0; v < CH; ++v)
+      {
+        const double w = static_cast<double>(w_g[v * 64 + lane]);
+        const double2 fp = fp_g[v];
+        XA = fma(w, fp.x, XA);
+        YA = fma(-w, fp.y, YA);
+      }
+    }
 
     for (int32_t blk = b_lo; blk < b_hi; blk += 64)
     {
@@ -675,31 +704,26 @@ namespace sbamd
       // skips the tie flags; a block with an edge takes the checked walk
       const bool untied = !ties_possible && !edge;
 
-      double X = 0.0, Y = 0.0, J = 0.0; // over the group's partners
+      // the row's sums at the block start (over the group's partners)
+      double X = __shfl(XA, ib);
+      double Y = __shfl(YA, ib);
+      double J = 0.0;
 #pragma unroll 1
       for (int g = 0; g < G; ++g)
       {
         if ((grp * G + g) * CH >= B.n)
           break;
         const WT* const w_g = s_w + g * CH * 64;
-        double2* const fp_g = s_fp + g * CH;
-        const int32_t* const oend_g = s_oend + g * CH;
         const BwdEvent *cl0, *cl1;
         int32_t cn0, cn1;
         list_of(g, 0, cl0, cn0);
         list_of(g, 1, cl1, cn1);
-        // partners at the block start; the lane's row of weights of each
-        // sublist (partner c at [64 c]), read from LDS event by event
+        // the lane's row of weights of each sublist (partner c at [64 c]),
+        // and its own row's (row = lane), read from LDS event by event
         const WT* const w_row0 = w_g + ib;
         const WT* const w_row1 = w_g + VW * 64 + ib;
-#pragma unroll
-        for (int v = 0; v < CH; ++v)
-        {
-          const WT w = w_g[v * 64 + ib];
-          const double2 fp = fp_g[v];
-          X = fma(static_cast<double>(w), fp.x, X);
-          Y = fma(-static_cast<double>(w), fp.y, Y);
-        }
+        const WT* const w_own0 = w_g + lane;
+        const WT* const w_own1 = w_g + VW * 64 + lane;
 
         uint64_t tie = 0;
         const int32_t ws0 = __builtin_amdgcn_readlane(ecv, 2 * g);
@@ -707,8 +731,8 @@ namespace sbamd
         int32_t we0, we1;
         if (untied)
         {
-          we0 = walk_window<WT, false>(cl0, cn0, ws0, lim, w_row0, tb, X, Y, tie);
-          we1 = walk_window<WT, false>(cl1, cn1, ws1, lim, w_row1, tb, X, Y, tie);
+          we0 = walk_window<WT, false>(cl0, cn0, ws0, lim, w_row0, w_own0, tb, X, Y, XA, YA, tie);
+          we1 = walk_window<WT, false>(cl1, cn1, ws1, lim, w_row1, w_own1, tb, X, Y, XA, YA, tie);
           // the one tie left: the lane's own PCF among the partners, whose
           // event at tb is the lane's breakpoint itself
           const int64_t js = band * BWD_GROUP + ib - (grp * G + g) * CH;
@@ -717,8 +741,8 @@ namespace sbamd
         }
         else
         {
-          we0 = walk_window<WT, true>(cl0, cn0, ws0, lim, w_row0, tb, X, Y, tie);
-          we1 = walk_window<WT, true>(cl1, cn1, ws1, lim, w_row1, tb, X, Y, tie);
+          we0 = walk_window<WT, true>(cl0, cn0, ws0, lim, w_row0, w_own0, tb, X, Y, XA, YA, tie);
+          we1 = walk_window<WT, true>(cl1, cn1, ws1, lim, w_row1, w_own1, tb, X, Y, XA, YA, tie);
         }
         ecv = lane == 2 * g ? we0 : (lane == 2 * g + 1 ? we1 : ecv);
 
@@ -740,31 +764,6 @@ namespace sbamd
               break;
             if (ev.t == tb)
               J = fma(static_cast<double>(w_row1[(ev.col & (VW - 1)) * 64]), ev.d, J);
-          }
-        }
-
-        if (!lastblk)
-        {
-          // move the partners' state to Tend: per partner, the last of its
-          // events in the window (the record that reaches Tend) sets it
-#pragma unroll
-          for (int s = 0; s < 2; ++s)
-          {
-            const BwdEvent* const lst = s ? cl1 : cl0;
-            const int32_t e1 = s ? we1 : we0;
-            for (int32_t e0 = s ? ws1 : ws0; e0 < e1; e0 += 64)
-            {
-              const int32_t e = e0 + lane;
-              if (e < e1)
-              {
-                const BwdEvent& ev = lst[e];
-                const int jj = s * VW + (ev.col & (VW - 1));
-                const int32_t p = ev.pos;
-                const double tn = p + 1 < oend_g[jj] ? B.key0[p + 1] : __builtin_inf();
-                if (tn >= Tend)
-                  fp_g[jj] = make_double2(B.val0[p], B.pst0[p]);
-              }
-            }
           }
         }
       }
