@@ -176,6 +176,15 @@ namespace sbamd
       return static_cast<int32_t>(r);
     }
 
+    // Lane u's double, as a wavefront-uniform value.
+    __device__ __forceinline__ double readlane_f64(double x, int u)
+    {
+      const uint64_t b = __builtin_bit_cast(uint64_t, x);
+      const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(b), u));
+      const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), u));
+      return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+    }
+
     // A partner event as the walk reads it: one 32-byte scalar load.
     struct WalkEv
     {
@@ -345,8 +354,9 @@ namespace sbamd
   }
 
   // Wavefronts per backward workgroup (slices of one task's breakpoints):
-  // with 57 vector registers a lane, three 8-wave workgroups share a CU (set
-  // by LDS), so one's set-up (weights, searches) overlaps the others' walks.
+  // with 59 vector registers a lane and 34.5 KB of LDS, four 8-wave
+  // workgroups share a CU, so one's set-up (weights, searches) overlaps the
+  // others' walks.
   // (With the weights in vector registers instead of LDS, 91 registers and
   // two workgroups per CU: 1.39-1.42 ms against 1.32-1.36 on 4096 PCFs.)
   constexpr int BWD_WAVES = 8;
@@ -358,8 +368,7 @@ namespace sbamd
 
   // LDS of the gather kernel, per chunk of the group: pair weights (ch
   // partners x 64 rows), the partners' record end positions (ch ints), and
-  // per wave the partners' state (f, P) at the current block's start (ch x
-  // 16 bytes).
+  // the partners' state (f, P) at the end of the domain (ch x 16 bytes).
   __host__ __device__ constexpr size_t bwd_lds_weights(size_t wbytes, int ch)
   {
     return static_cast<size_t>(ch) * 64 * wbytes;
@@ -367,7 +376,7 @@ namespace sbamd
   __host__ __device__ constexpr size_t bwd_lds_bytes(size_t wbytes, int ch)
   {
     return BWD_CHUNKS * (bwd_lds_weights(wbytes, ch) + static_cast<size_t>(ch) * 4 +
-                         BWD_WAVES * static_cast<size_t>(ch) * 16);
+                         static_cast<size_t>(ch) * 16);
   }
 
   // Thread per sorted record: copies start time, value, owner and source
@@ -523,9 +532,8 @@ namespace sbamd
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    // this wave's partner states (f, P), CH per chunk of the group
-    double2* const s_fp =
-        reinterpret_cast<double2*>(smem + G * bwd_lds_weights(sizeof(WT), CH) + G * CH * 4) + wv * G * CH;
+    // the partners' states (f, P) at the end of the domain, CH per chunk
+    double2* const s_fp = reinterpret_cast<double2*>(smem + G * bwd_lds_weights(sizeof(WT), CH) + G * CH * 4);
     const int64_t nbands = (B.n + BWD_GROUP - 1) / BWD_GROUP;
     const int64_t nchunks = (B.n + CH - 1) / CH;
     const int64_t ngroups = (nchunks + G - 1) / G;
@@ -630,24 +638,7 @@ namespace sbamd
       }
     };
 
-    // partners' state just before the slice's first breakpoint
     const double T0 = uniform_load(&rl[b_lo].t, 0);
-    for (int k = lane; k < G * CH; k += 64)
-    {
-      double f = 0.0, P = 0.0;
-      const int64_t jg = grp * G * CH + k;
-      if (jg < B.n)
-      {
-        const int64_t a = B.ooff[jg];
-        const int64_t c = starts_below(B.key0, a, s_oend[k], T0);
-        if (c > 0)
-        {
-          f = B.val0[a + c - 1];
-          P = B.pst0[a + c - 1];
-        }
-      }
-      s_fp[k] = make_double2(f, P);
-    }
     // the walk's position in each list: lane 2 g + s of one vector register
     int32_t ecv = 0;
     for (int k = 0; k < 2 * G; ++k)
@@ -661,22 +652,38 @@ namespace sbamd
     __builtin_amdgcn_wave_barrier();
     // Row `lane`'s sums over the group's partners at the current block start,
     // X = sum_j w f_j and Y = -sum_j w P_j: formed once from the partners'
-    // state at the slice start, then moved along by every event the walk
-    // reads; a block's lanes take their rows' sums from these lanes.
+    // state just before the slice's first breakpoint (lane l finds partner
+    // k0 + l's, every lane takes it from there), then moved along by every
+    // event the walk reads; a block's lanes take their rows' sums from these
+    // lanes.
     double XA = 0.0, YA = 0.0;
     for (int g = 0; g < G; ++g)
     {
       if ((grp * G + g) * CH >= B.n)
         break;
       const WT* const w_g = s_w + g * CH * 64;
-      const double2* const fp_g = s_fp + g * CH;
-#pragma unroll 8
-      for (int v = 0; v < CH; ++v)
+      for (int k0 = 0; k0 < CH; k0 += 64)
       {
-        const double w = static_cast<double>(w_g[v * 64 + lane]);
-        const double2 fp = fp_g[v];
-        XA = fma(w, fp.x, XA);
-        YA = fma(-w, fp.y, YA);
+        double f = 0.0, P = 0.0;
+        const int k = k0 + lane;
+        const int64_t jg = (grp * G + g) * CH + k;
+        if (k < CH && jg < B.n)
+        {
+          const int64_t a = B.ooff[jg];
+          const int64_t c = starts_below(B.key0, a, s_oend[g * CH + k], T0);
+          if (c > 0)
+          {
+            f = B.val0[a + c - 1];
+            P = B.pst0[a + c - 1];
+          }
+        }
+        const int nk = min(64, CH - k0);
+        for (int u = 0; u < nk; ++u)
+        {
+          const double w = static_cast<double>(w_g[(k0 + u) * 64 + lane]);
+          XA = fma(w, readlane_f64(f, u), XA);
+          YA = fma(-w, readlane_f64(P, u), YA);
+        }
       }
     }
 
