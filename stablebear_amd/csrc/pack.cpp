@@ -317,8 +317,11 @@ namespace sbamd
   }
 
   // Reorders tiles [b0, b1) of a launch with one workgroup per tile so that
-  // each XCD works through supertiles, blocks of G x G tiles that share their
-  // R * G rows and C * G columns, and re-reads those PCFs from its own L2.
+  // each XCD works through supertiles, blocks of tiles that share their rows
+  // and columns, and re-reads those PCFs from its own L2. A supertile of size
+  // G is 2G tiles tall and G/2 wide (G x G for G = 4): on the 20,000-PCF L1
+  // pdist with 16 x 32 tiles, 512 rows x 256 columns of PCFs took 41.5 ms
+  // against 42.5 for 256 x 512 (16 x 16 tiles) and 41.7 for 256 x 256.
   // The dispatcher hands workgroup w to XCD w mod nxcd, so XCD x runs the
   // tiles at positions congruent to x: the supertiles, taken in the order of
   // their first tile (the longest first, as the list was), are dealt to the
@@ -334,7 +337,8 @@ namespace sbamd
       return 0;
     const size_t n = b1 - b0;
     auto key = [&](const Tile& t, int G) {
-      const int sr = G * R, sc = G * C; // powers of two for the tile sides in use: shifts
+      const int sr = (G > 4 ? 2 * G : G) * R, sc = (G > 4 ? G / 2 : G) * C;
+      // powers of two for the tile sides in use: shifts
       const uint64_t r = (sr & (sr - 1)) == 0 ? static_cast<uint64_t>(t.r0) >> __builtin_ctz(sr)
                                               : static_cast<uint64_t>(t.r0 / sr);
       const uint64_t c = (sc & (sc - 1)) == 0 ? static_cast<uint64_t>(t.c0) >> __builtin_ctz(sc)
